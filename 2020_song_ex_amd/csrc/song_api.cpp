@@ -1,0 +1,512 @@
+// song_api.cpp -- the C ABI of libsong.so (declared in include/song.h):
+// index residency, the batched search driver, graph files and device-memory
+// helpers.  The reference's counterpart is the host half of kernalGraph.h /
+// hashKernelGraph.h (upload_to_gpu, gpu_search).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/song.h"
+#include "nsw_build.h"
+#include "song_kernels.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+    g_err = std::string(what) + ": " + hipGetErrorString(e);
+    return SONG_ERR_HIP;
+}
+
+#define HIP_TRY(expr, what)                          \
+    do {                                             \
+        hipError_t e_ = (expr);                      \
+        if (e_ != hipSuccess) return hip_fail(e_, what); \
+    } while (0)
+
+double ms_since(std::chrono::steady_clock::time_point a) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
+}
+
+bool is_pow2(int64_t v) { return v > 0 && (v & (v - 1)) == 0; }
+
+int ilog2(int64_t v) {
+    int r = 0;
+    while ((int64_t(1) << r) < v) r++;
+    return r;
+}
+
+}  // namespace
+
+struct song_index {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int metric = 0;
+    int64_t n = 0;
+    int dim = 0;      // floats per vector (256 for Hamming bits)
+    int dstride = 0;  // floats (or u64) per stored row
+    int block = 0;
+    void* d_data = nullptr;
+    int32_t* d_adj = nullptr;
+    // grow-only workspace for host-buffer searches
+    void* d_q = nullptr;
+    int32_t* d_ids = nullptr;
+    float* d_dists = nullptr;
+    int32_t* d_trace = nullptr;
+    int64_t cap_q = 0, cap_ids = 0, cap_trace = 0;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::vector<char> stage;  // host staging for padded query rows
+    std::mutex mu;
+};
+
+namespace {
+
+size_t row_bytes(const song_index* ix) { return (size_t)ix->dstride * (ix->metric == SONG_METRIC_HAMMING ? 8 : 4); }
+size_t host_row_bytes(const song_index* ix) {
+    return ix->metric == SONG_METRIC_HAMMING ? 32 : (size_t)ix->dim * 4;
+}
+
+struct Plan {
+    int ef, S, G, qch, wpb, bloom_words, bloom_shift, n_seeds, batch;
+};
+
+int make_plan(const song_index* ix, int32_t nq, int32_t k, int32_t ef, const song_search_opts* o, Plan* p) {
+    song_search_opts z;
+    std::memset(&z, 0, sizeof z);
+    if (!o) o = &z;
+    if (nq < 0) return fail(SONG_ERR_ARG, "nq must be >= 0");
+    if (k < 1) return fail(SONG_ERR_ARG, "k must be >= 1");
+    if (ef < k) ef = k;  // kernalGraph.h:315
+    if (ef > 512) return fail(SONG_ERR_ARG, "ef (queue size) must be <= 512 after ef = max(ef, k)");
+    p->ef = ef;
+    p->S = ef <= 64 ? 1 : ef <= 128 ? 2 : ef <= 256 ? 4 : 8;
+    int64_t bits = o->bloom_bits ? o->bloom_bits : 16384;
+    if (!is_pow2(bits) || bits < 512 || bits > 65536)
+        return fail(SONG_ERR_ARG, "bloom_bits must be a power of two in [512, 65536]");
+    p->bloom_words = (int)(bits / 64);
+    p->bloom_shift = ilog2(p->bloom_words);
+    p->n_seeds = o->n_seeds ? o->n_seeds : 4;
+    if (p->n_seeds < 1 || p->n_seeds > 64) return fail(SONG_ERR_ARG, "n_seeds must be in [1, 64]");
+    p->wpb = o->waves_per_block ? o->waves_per_block : 4;
+    if (p->wpb != 1 && p->wpb != 2 && p->wpb != 4) return fail(SONG_ERR_ARG, "waves_per_block must be 1, 2 or 4");
+    p->batch = o->batch > 0 ? o->batch : (nq > 0 ? nq : 1);
+    if (o->batch < 0) return fail(SONG_ERR_ARG, "batch must be >= 0");
+    if (ix->metric == SONG_METRIC_HAMMING) {
+        p->G = 2;
+        p->qch = 1;
+    } else {
+        const int d4 = ix->dstride / 4;
+        int G = o->lanes_per_row;
+        if (G) {
+            if (!is_pow2(G) || G < 2 || G > 64) return fail(SONG_ERR_ARG, "lanes_per_row must be a power of two in [2, 64]");
+        } else {
+            G = 2;
+            while (G < 64 && G * 4 < d4) G *= 2;
+        }
+        int qch = 4;
+        while (G * qch < d4) qch *= 2;
+        if (qch > 4 && G != 64) return fail(SONG_ERR_ARG, "lanes_per_row too small for this dim");
+        if (qch > 16) return fail(SONG_ERR_ARG, "dim > 4096 is not supported");
+        p->G = G;
+        p->qch = qch;
+    }
+    return SONG_OK;
+}
+
+song::SearchArgs make_args(const song_index* ix, const Plan& p, const void* dq, int32_t nq, int32_t k,
+                           int32_t* ids, float* dists, int32_t* trace) {
+    song::SearchArgs a;
+    a.data = ix->d_data;
+    a.adj = ix->d_adj;
+    a.queries = dq;
+    a.ids_out = ids;
+    a.dists_out = dists;
+    a.trace_out = trace;
+    a.entries = nullptr;
+    a.n = ix->n;
+    a.dstride = ix->dstride;
+    a.block = ix->block;
+    a.k = k;
+    a.ef = p.ef;
+    a.nq = nq;
+    a.bloom_words = p.bloom_words;
+    a.bloom_shift = p.bloom_shift;
+    a.n_entry = p.n_seeds;
+    return a;
+}
+
+int ensure_ws(song_index* ix, int64_t nq, int32_t k, bool dists, bool trace, double* alloc_ms) {
+    auto t0 = std::chrono::steady_clock::now();
+    const int64_t need_q = nq * (int64_t)row_bytes(ix);
+    if (need_q > ix->cap_q) {
+        if (ix->d_q) (void)hipFree(ix->d_q);
+        ix->d_q = nullptr;
+        ix->cap_q = 0;
+        HIP_TRY(hipMalloc(&ix->d_q, need_q), "hipMalloc queries");
+        ix->cap_q = need_q;
+    }
+    const int64_t need_ids = nq * (int64_t)k;
+    if (need_ids > ix->cap_ids) {
+        if (ix->d_ids) (void)hipFree(ix->d_ids);
+        if (ix->d_dists) (void)hipFree(ix->d_dists);
+        ix->d_ids = nullptr;
+        ix->d_dists = nullptr;
+        ix->cap_ids = 0;
+        HIP_TRY(hipMalloc(&ix->d_ids, need_ids * 4), "hipMalloc ids");
+        HIP_TRY(hipMalloc(&ix->d_dists, need_ids * 4), "hipMalloc dists");
+        ix->cap_ids = need_ids;
+    }
+    if (trace && nq > ix->cap_trace) {
+        if (ix->d_trace) (void)hipFree(ix->d_trace);
+        ix->d_trace = nullptr;
+        ix->cap_trace = 0;
+        HIP_TRY(hipMalloc(&ix->d_trace, nq * 12 * 4), "hipMalloc trace");
+        ix->cap_trace = nq;
+    }
+    (void)dists;
+    if (alloc_ms) *alloc_ms = ms_since(t0);
+    return SONG_OK;
+}
+
+// Pads host rows (dim floats) to the device row stride.
+const void* stage_rows(song_index* ix, const void* src, int64_t rows) {
+    const size_t hb = host_row_bytes(ix), db = row_bytes(ix);
+    if (hb == db) return src;
+    ix->stage.assign((size_t)rows * db, 0);
+    for (int64_t r = 0; r < rows; r++) std::memcpy(ix->stage.data() + r * db, (const char*)src + r * hb, hb);
+    return ix->stage.data();
+}
+
+}  // namespace
+
+extern "C" {
+
+int song_abi_version(void) { return SONG_ABI_VERSION; }
+
+const char* song_last_error(void) { return g_err.c_str(); }
+
+int song_device_count(int32_t* out) {
+    if (!out) return fail(SONG_ERR_ARG, "null out");
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    if (e != hipSuccess) c = 0;
+    *out = c;
+    return SONG_OK;
+}
+
+int song_nsw_build(const void* vecs, int64_t n, int32_t dim, int32_t metric, int32_t block, int32_t fixed_deg,
+                   int32_t search_deg, int32_t ef_construction, int32_t n_threads, int32_t batch_div,
+                   int64_t seq_prefix, int32_t exact_order, int32_t verbose, int32_t* adj_out) {
+    if (!vecs || !adj_out) return fail(SONG_ERR_ARG, "null buffer");
+    if (n < 1 || n > INT32_MAX) return fail(SONG_ERR_ARG, "n must be in [1, 2^31)");
+    if (metric < 0 || metric > 2) return fail(SONG_ERR_ARG, "unknown metric");
+    if (metric != SONG_METRIC_HAMMING && dim < 1) return fail(SONG_ERR_ARG, "dim must be >= 1");
+    if (block < 2 || block > 64) return fail(SONG_ERR_ARG, "block must be in [2, 64]");
+    if (fixed_deg < 1 || fixed_deg >= block) return fail(SONG_ERR_ARG, "fixed_deg must be in [1, block)");
+    if (search_deg < 1 || search_deg > fixed_deg) return fail(SONG_ERR_ARG, "search_deg must be in [1, fixed_deg]");
+    if (ef_construction < 1) return fail(SONG_ERR_ARG, "ef_construction must be >= 1");
+    if (batch_div < 0) return fail(SONG_ERR_ARG, "batch_div must be >= 0");
+    song::BuildParams p;
+    p.n = n;
+    p.dim = metric == SONG_METRIC_HAMMING ? 256 : dim;
+    p.metric = metric;
+    p.block = block;
+    p.fixed_deg = fixed_deg;
+    p.search_deg = search_deg;
+    p.ef_construction = ef_construction;
+    p.n_threads = n_threads < 1 ? 1 : n_threads;
+    p.batch_div = batch_div;
+    p.seq_prefix = seq_prefix < 1 ? 1 : seq_prefix;
+    p.exact_order = exact_order;
+    p.verbose = verbose;
+    try {
+        return song::nsw_build(vecs, p, adj_out);
+    } catch (const std::bad_alloc&) {
+        return fail(SONG_ERR_ALLOC, "host allocation failed in nsw_build");
+    }
+}
+
+// Graph file: "SONGGRF1", int64 n, int32 block, metric, dim, reserved, then
+// int32 adjacency [n x block].
+int song_graph_save(const char* path, const int32_t* adj, int64_t n, int32_t block, int32_t metric, int32_t dim) {
+    if (!path || !adj || n < 1 || block < 1) return fail(SONG_ERR_ARG, "bad graph_save arguments");
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return fail(SONG_ERR_IO, std::string("cannot open ") + path);
+    const char magic[8] = {'S', 'O', 'N', 'G', 'G', 'R', 'F', '1'};
+    int32_t hdr[4] = {block, metric, dim, 0};
+    bool ok = std::fwrite(magic, 1, 8, f) == 8 && std::fwrite(&n, 8, 1, f) == 1 && std::fwrite(hdr, 4, 4, f) == 4 &&
+              std::fwrite(adj, 4, (size_t)(n * block), f) == (size_t)(n * block);
+    ok = (std::fclose(f) == 0) && ok;
+    return ok ? SONG_OK : fail(SONG_ERR_IO, std::string("write failed: ") + path);
+}
+
+int song_graph_load_header(const char* path, int64_t* n, int32_t* block, int32_t* metric, int32_t* dim) {
+    if (!path || !n || !block || !metric || !dim) return fail(SONG_ERR_ARG, "null argument");
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return fail(SONG_ERR_IO, std::string("cannot open ") + path);
+    char magic[8];
+    int32_t hdr[4];
+    bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, "SONGGRF1", 8) == 0 &&
+              std::fread(n, 8, 1, f) == 1 && std::fread(hdr, 4, 4, f) == 4;
+    std::fclose(f);
+    if (!ok) return fail(SONG_ERR_IO, std::string("not a SONG graph file: ") + path);
+    *block = hdr[0];
+    *metric = hdr[1];
+    *dim = hdr[2];
+    return SONG_OK;
+}
+
+int song_graph_load(const char* path, int32_t* adj_out, int64_t n, int32_t block) {
+    int64_t fn;
+    int32_t fb, fm, fd;
+    int rc = song_graph_load_header(path, &fn, &fb, &fm, &fd);
+    if (rc) return rc;
+    if (!adj_out || fn != n || fb != block) return fail(SONG_ERR_ARG, "graph file shape differs from the request");
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return fail(SONG_ERR_IO, std::string("cannot open ") + path);
+    bool ok = std::fseek(f, 8 + 8 + 16, SEEK_SET) == 0 &&
+              std::fread(adj_out, 4, (size_t)(n * block), f) == (size_t)(n * block);
+    std::fclose(f);
+    return ok ? SONG_OK : fail(SONG_ERR_IO, std::string("short read: ") + path);
+}
+
+int song_index_create(const void* vecs, const int32_t* adj, int64_t n, int32_t dim, int32_t block, int32_t metric,
+                      int32_t device, song_index** out) {
+    if (!out) return fail(SONG_ERR_ARG, "null out");
+    *out = nullptr;
+    if (!vecs || !adj) return fail(SONG_ERR_ARG, "null buffer");
+    if (n < 1 || n > INT32_MAX) return fail(SONG_ERR_ARG, "n must be in [1, 2^31)");
+    if (metric < 0 || metric > 2) return fail(SONG_ERR_ARG, "unknown metric");
+    if (block < 1 || block > 64) return fail(SONG_ERR_ARG, "block must be in [1, 64]");
+    if (metric == SONG_METRIC_HAMMING) {
+        if (dim != 256) return fail(SONG_ERR_ARG, "Hamming codes must have dim = 256 bits");
+    } else if (dim < 1 || dim > 4096) {
+        return fail(SONG_ERR_ARG, "dim must be in [1, 4096]");
+    }
+    for (int64_t i = 0; i < n * block; i++) {
+        if (adj[i] < -1 || adj[i] >= n) return fail(SONG_ERR_ARG, "adjacency entry out of range at " + std::to_string(i));
+    }
+    int nd = 0;
+    HIP_TRY(hipGetDeviceCount(&nd), "hipGetDeviceCount");
+    if (device < 0 || device >= nd) return fail(SONG_ERR_ARG, "device out of range");
+    HIP_TRY(hipSetDevice(device), "hipSetDevice");
+    song_index* ix = new song_index();
+    ix->device = device;
+    ix->metric = metric;
+    ix->n = n;
+    ix->dim = dim;
+    ix->dstride = metric == SONG_METRIC_HAMMING ? 4 : (dim + 3) / 4 * 4;
+    ix->block = block;
+    auto bail = [&](int rc) {
+        song_index_destroy(ix);
+        return rc;
+    };
+    hipError_t e = hipStreamCreateWithFlags(&ix->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return bail(hip_fail(e, "hipStreamCreate"));
+    for (auto& ev : ix->ev) {
+        e = hipEventCreate(&ev);
+        if (e != hipSuccess) return bail(hip_fail(e, "hipEventCreate"));
+    }
+    const size_t rb = row_bytes(ix);
+    e = hipMalloc(&ix->d_data, (size_t)n * rb);
+    if (e != hipSuccess) return bail(hip_fail(e, "hipMalloc vectors"));
+    e = hipMalloc(&ix->d_adj, (size_t)n * block * 4);
+    if (e != hipSuccess) return bail(hip_fail(e, "hipMalloc adjacency"));
+    const size_t hb = host_row_bytes(ix);
+    if (hb == rb) {
+        e = hipMemcpy(ix->d_data, vecs, (size_t)n * rb, hipMemcpyHostToDevice);
+    } else {
+        // pad rows in chunks
+        const int64_t chunk = 65536;
+        std::vector<char> buf((size_t)chunk * rb, 0);
+        for (int64_t r0 = 0; r0 < n && e == hipSuccess; r0 += chunk) {
+            const int64_t rows = std::min(chunk, n - r0);
+            for (int64_t r = 0; r < rows; r++)
+                std::memcpy(buf.data() + r * rb, (const char*)vecs + (r0 + r) * hb, hb);
+            e = hipMemcpy((char*)ix->d_data + r0 * rb, buf.data(), (size_t)rows * rb, hipMemcpyHostToDevice);
+        }
+    }
+    if (e != hipSuccess) return bail(hip_fail(e, "upload vectors"));
+    e = hipMemcpy(ix->d_adj, adj, (size_t)n * block * 4, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return bail(hip_fail(e, "upload adjacency"));
+    *out = ix;
+    return SONG_OK;
+}
+
+void song_index_destroy(song_index* ix) {
+    if (!ix) return;
+    (void)hipSetDevice(ix->device);
+    if (ix->stream) (void)hipStreamSynchronize(ix->stream);
+    if (ix->d_data) (void)hipFree(ix->d_data);
+    if (ix->d_adj) (void)hipFree(ix->d_adj);
+    if (ix->d_q) (void)hipFree(ix->d_q);
+    if (ix->d_ids) (void)hipFree(ix->d_ids);
+    if (ix->d_dists) (void)hipFree(ix->d_dists);
+    if (ix->d_trace) (void)hipFree(ix->d_trace);
+    for (auto& ev : ix->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (ix->stream) (void)hipStreamDestroy(ix->stream);
+    delete ix;
+}
+
+int song_index_row_stride(const song_index* ix, int32_t* out) {
+    if (!ix || !out) return fail(SONG_ERR_ARG, "null argument");
+    *out = ix->dstride;
+    return SONG_OK;
+}
+
+int song_search(song_index* ix, const void* queries, int32_t nq, int32_t k, int32_t ef, const song_search_opts* opts,
+                int32_t* ids_out, float* dists_out, song_query_trace* trace_out, song_timing* timing_out,
+                song_batch_timing* batch_out) {
+    if (!ix) return fail(SONG_ERR_ARG, "null index");
+    if (nq > 0 && (!queries || !ids_out)) return fail(SONG_ERR_ARG, "null buffer");
+    Plan p;
+    int rc = make_plan(ix, nq, k, ef, opts, &p);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lock(ix->mu);
+    HIP_TRY(hipSetDevice(ix->device), "hipSetDevice");
+    song_timing tm;
+    std::memset(&tm, 0, sizeof tm);
+    tm.ef_used = p.ef;
+    const int64_t bmax = std::min<int64_t>(p.batch, std::max<int32_t>(nq, 1));
+    rc = ensure_ws(ix, bmax, k, dists_out != nullptr, trace_out != nullptr, &tm.alloc_ms);
+    if (rc) return rc;
+    const size_t hb = host_row_bytes(ix), rb = row_bytes(ix);
+    auto t0 = std::chrono::steady_clock::now();
+    int bi = 0;
+    for (int64_t off = 0; off < nq; off += p.batch, bi++) {
+        const int32_t bs = (int32_t)std::min<int64_t>(p.batch, nq - off);
+        const void* src = stage_rows(ix, (const char*)queries + off * hb, bs);
+        HIP_TRY(hipEventRecord(ix->ev[0], ix->stream), "hipEventRecord");
+        HIP_TRY(hipMemcpyAsync(ix->d_q, src, (size_t)bs * rb, hipMemcpyHostToDevice, ix->stream), "query H2D");
+        HIP_TRY(hipEventRecord(ix->ev[1], ix->stream), "hipEventRecord");
+        song::SearchArgs a = make_args(ix, p, ix->d_q, bs, k, ix->d_ids, dists_out ? ix->d_dists : nullptr,
+                                       trace_out ? ix->d_trace : nullptr);
+        HIP_TRY(song::launch_search(a, ix->metric, p.S, p.G, p.qch, p.wpb, ix->stream), "search kernel launch");
+        HIP_TRY(hipEventRecord(ix->ev[2], ix->stream), "hipEventRecord");
+        HIP_TRY(hipMemcpyAsync(ids_out + off * k, ix->d_ids, (size_t)bs * k * 4, hipMemcpyDeviceToHost, ix->stream),
+                "ids D2H");
+        if (dists_out)
+            HIP_TRY(hipMemcpyAsync(dists_out + off * k, ix->d_dists, (size_t)bs * k * 4, hipMemcpyDeviceToHost,
+                                   ix->stream),
+                    "dists D2H");
+        if (trace_out)
+            HIP_TRY(hipMemcpyAsync(trace_out + off, ix->d_trace, (size_t)bs * 48, hipMemcpyDeviceToHost, ix->stream),
+                    "trace D2H");
+        HIP_TRY(hipEventRecord(ix->ev[3], ix->stream), "hipEventRecord");
+        HIP_TRY(hipEventSynchronize(ix->ev[3]), "search synchronize");
+        float h2d = 0, ker = 0, d2h = 0;
+        HIP_TRY(hipEventElapsedTime(&h2d, ix->ev[0], ix->ev[1]), "hipEventElapsedTime");
+        HIP_TRY(hipEventElapsedTime(&ker, ix->ev[1], ix->ev[2]), "hipEventElapsedTime");
+        HIP_TRY(hipEventElapsedTime(&d2h, ix->ev[2], ix->ev[3]), "hipEventElapsedTime");
+        tm.h2d_ms += h2d;
+        tm.kernel_ms += ker;
+        tm.d2h_ms += d2h;
+        if (batch_out) {
+            song_batch_timing& b = batch_out[bi];
+            b.batch_index = bi;
+            b.batch_offset = (int32_t)off;
+            b.batch_size = bs;
+            b.pad_ = 0;
+            b.h2d_ms = h2d;
+            b.memset_ms = 0.0;
+            b.kernel_ms = ker;
+            b.d2h_ms = d2h;
+        }
+    }
+    tm.total_ms = ms_since(t0);
+    tm.n_batches = bi;
+    if (timing_out) *timing_out = tm;
+    return SONG_OK;
+}
+
+int song_search_device(song_index* ix, const void* d_queries, int32_t nq, int32_t k, int32_t ef,
+                       const song_search_opts* opts, int32_t* d_ids, float* d_dists, song_query_trace* d_trace,
+                       song_timing* timing_out) {
+    if (!ix) return fail(SONG_ERR_ARG, "null index");
+    if (nq > 0 && (!d_queries || !d_ids)) return fail(SONG_ERR_ARG, "null buffer");
+    Plan p;
+    int rc = make_plan(ix, nq, k, ef, opts, &p);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lock(ix->mu);
+    HIP_TRY(hipSetDevice(ix->device), "hipSetDevice");
+    song_timing tm;
+    std::memset(&tm, 0, sizeof tm);
+    tm.ef_used = p.ef;
+    const size_t rb = row_bytes(ix);
+    auto t0 = std::chrono::steady_clock::now();
+    HIP_TRY(hipEventRecord(ix->ev[1], ix->stream), "hipEventRecord");
+    int bi = 0;
+    for (int64_t off = 0; off < nq; off += p.batch, bi++) {
+        const int32_t bs = (int32_t)std::min<int64_t>(p.batch, nq - off);
+        song::SearchArgs a = make_args(ix, p, (const char*)d_queries + off * rb, bs, k, d_ids + off * k,
+                                       d_dists ? d_dists + off * k : nullptr,
+                                       d_trace ? (int32_t*)(d_trace + off) : nullptr);
+        HIP_TRY(song::launch_search(a, ix->metric, p.S, p.G, p.qch, p.wpb, ix->stream), "search kernel launch");
+    }
+    HIP_TRY(hipEventRecord(ix->ev[2], ix->stream), "hipEventRecord");
+    HIP_TRY(hipEventSynchronize(ix->ev[2]), "search synchronize");
+    float ker = 0;
+    HIP_TRY(hipEventElapsedTime(&ker, ix->ev[1], ix->ev[2]), "hipEventElapsedTime");
+    tm.kernel_ms = ker;
+    tm.total_ms = ms_since(t0);
+    tm.n_batches = bi;
+    if (timing_out) *timing_out = tm;
+    return SONG_OK;
+}
+
+int song_dev_alloc(int32_t device, int64_t bytes, void** out) {
+    if (!out || bytes < 0) return fail(SONG_ERR_ARG, "bad dev_alloc arguments");
+    *out = nullptr;
+    HIP_TRY(hipSetDevice(device), "hipSetDevice");
+    HIP_TRY(hipMalloc(out, bytes > 0 ? bytes : 1), "hipMalloc");
+    return SONG_OK;
+}
+
+int song_dev_free(void* p) {
+    if (p) HIP_TRY(hipFree(p), "hipFree");
+    return SONG_OK;
+}
+
+int song_memcpy_h2d(void* dst, const void* src, int64_t bytes) {
+    if ((!dst || !src) && bytes > 0) return fail(SONG_ERR_ARG, "null buffer");
+    if (bytes > 0) HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
+    return SONG_OK;
+}
+
+int song_memcpy_d2h(void* dst, const void* src, int64_t bytes) {
+    if ((!dst || !src) && bytes > 0) return fail(SONG_ERR_ARG, "null buffer");
+    if (bytes > 0) HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    return SONG_OK;
+}
+
+int song_queries_to_device(song_index* ix, const void* queries, int32_t nq, void** d_out) {
+    if (!ix || !d_out || (nq > 0 && !queries) || nq < 0) return fail(SONG_ERR_ARG, "bad arguments");
+    std::lock_guard<std::mutex> lock(ix->mu);
+    HIP_TRY(hipSetDevice(ix->device), "hipSetDevice");
+    const size_t rb = row_bytes(ix);
+    *d_out = nullptr;
+    HIP_TRY(hipMalloc(d_out, std::max<size_t>(1, (size_t)nq * rb)), "hipMalloc queries");
+    const void* src = stage_rows(ix, queries, nq);
+    hipError_t e = hipMemcpy(*d_out, src, (size_t)nq * rb, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(*d_out);
+        *d_out = nullptr;
+        return hip_fail(e, "queries H2D");
+    }
+    return SONG_OK;
+}
+
+}  // extern "C"

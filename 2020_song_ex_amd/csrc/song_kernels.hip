@@ -1,0 +1,591 @@
+// song_kernels.hip -- the SONG graph-search kernels for MI355X (gfx950).
+//
+// One 64-lane wavefront searches one query.  The algorithm is the reference's
+// kg_search_kernel / hkg_search_kernel (kernalGraph.h:184-299,
+// hashKernelGraph.h:238-352) with the same results bit for bit; only the
+// execution is re-planned for a wave64 machine:
+//
+//  * Candidates / Results (gpuHeap.h:8-109) keep the reference's
+//    unsorted-array semantics -- append while not full, otherwise replace the
+//    first-index maximum if strictly better; pop = first-index minimum, hole
+//    filled by the last element.  Entry i lives in VGPR slot i/64 of lane i%64,
+//    so every scan is a wave reduction (DPP max/min + 4 readlanes) followed by
+//    one ballot per slot to find the first index holding that value.
+//  * The Bloom visited set (bloomFilter.h:14-49, same hash, same word/bit
+//    layout) lives in LDS.  All slots of an adjacency row are hashed, tested
+//    and set lane-parallel; the reference's in-row order (a bit set by slot j
+//    is seen by slot j' > j) is honoured exactly: a 64-bit ds_or_rtn reveals
+//    any position two fresh slots share, and only then the row is replayed
+//    slot by slot.
+//  * Neighbour rows are gathered with 16-byte loads, G lanes per row, and the
+//    partial sums reduced by an xor butterfly built from DPP / swizzle.  The
+//    summation order (per-lane chunk order, explicit fmaf, butterfly) is fixed
+//    and is restated by the oracle.
+//  * The reference's ordered per-neighbour pushes become one batched step:
+//    appends are a lane rotation, and the remaining pushes on a full set are
+//    resolved in order by ballots -- only pushes that are accepted cost a
+//    reduction, rejected ones are skipped in bulk.
+//
+// Nothing here allocates; every buffer is passed in.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "song_kernels.h"
+
+namespace song {
+
+// ---------------------------------------------------------------- wave helpers
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+template <int CTRL>
+__device__ __forceinline__ float dppf(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ int dppi(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+
+constexpr int DPP_XOR1 = 0xB1;         // quad_perm [1,0,3,2]
+constexpr int DPP_XOR2 = 0x4E;         // quad_perm [2,3,0,1]
+constexpr int DPP_HALF_MIRROR = 0x141; // lane i <-> 7-i within 8
+constexpr int DPP_MIRROR = 0x140;      // lane i <-> 15-i within 16
+
+__device__ __forceinline__ float readlane_f(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+    v = fmaxf(v, dppf<DPP_XOR1>(v));
+    v = fmaxf(v, dppf<DPP_XOR2>(v));
+    v = fmaxf(v, dppf<DPP_HALF_MIRROR>(v));
+    v = fmaxf(v, dppf<DPP_MIRROR>(v));
+    float a = fmaxf(readlane_f(v, 0), readlane_f(v, 16));
+    float b = fmaxf(readlane_f(v, 32), readlane_f(v, 48));
+    return fmaxf(a, b);
+}
+
+__device__ __forceinline__ float wave_min(float v) {
+    v = fminf(v, dppf<DPP_XOR1>(v));
+    v = fminf(v, dppf<DPP_XOR2>(v));
+    v = fminf(v, dppf<DPP_HALF_MIRROR>(v));
+    v = fminf(v, dppf<DPP_MIRROR>(v));
+    float a = fminf(readlane_f(v, 0), readlane_f(v, 16));
+    float b = fminf(readlane_f(v, 32), readlane_f(v, 48));
+    return fminf(a, b);
+}
+
+// Sum over aligned groups of G lanes, xor-butterfly order: for s = 1, 2, ...
+// < G every lane t takes acc[t] + acc[t ^ s].  Steps 4 and 8 use the mirror
+// patterns, whose partner lies in the same already-uniform 4 / 8 block as the
+// xor partner, so the value is identical.  Every lane of the group ends with
+// the group's sum.
+template <int G>
+__device__ __forceinline__ float group_sum(float v) {
+    if constexpr (G > 1) v = v + dppf<DPP_XOR1>(v);
+    if constexpr (G > 2) v = v + dppf<DPP_XOR2>(v);
+    if constexpr (G > 4) v = v + dppf<DPP_HALF_MIRROR>(v);
+    if constexpr (G > 8) v = v + dppf<DPP_MIRROR>(v);
+    if constexpr (G > 16)
+        v = v + __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x1F | (16 << 10)));
+    if constexpr (G > 32) v = v + __shfl_xor(v, 32, 64);
+    return v;
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ int ffs64(uint64_t m) { return __ffsll((unsigned long long)m) - 1; }
+
+// ---------------------------------------------------------------- Bloom filter
+
+// BF_RAND, bloomFilter.h:14-19.
+__device__ __forceinline__ uint32_t bloom_pos(int h, uint32_t id, uint32_t nbits_mask) {
+    constexpr uint64_t R[8] = {
+        0x4bcb391f924ed183ULL, 0xa0ab69ccd854fc0aULL, 0x91086b9cecf5e3b7ULL, 0xc68e01641bead407ULL,
+        0x3a7b976128a30449ULL, 0x6d122efabfc4d99fULL, 0xe6700ef8715030e2ULL, 0x80dd0c3bffcfb45bULL,
+    };
+    // bf_hash, bloomFilter.h:23-30.  id < 2^31, so the first x ^= x >> 33 is
+    // the identity; the bit count is a power of two, so "% nbits" is a mask.
+    uint64_t x = (uint64_t)id;
+    x *= R[2 * h];
+    x ^= x >> 33;
+    x *= R[2 * h + 1];
+    x ^= x >> 33;
+    return (uint32_t)x & nbits_mask;
+}
+
+// Test-then-set of one row of ids in slot order (the reference's per-slot
+// bf_test / bf_set, kernalGraph.h:262-272).  Returns the mask of lanes whose
+// id was not (possibly) visited; their bits are now set.
+__device__ __forceinline__ uint64_t bloom_row(unsigned long long* bloom, uint32_t wmask, int wshift,
+                                              uint32_t nbits_mask, int id, bool valid) {
+    uint32_t word[4], bitn[4];
+    unsigned long long pre[4];
+    bool miss[4];
+    bool all = valid;
+#pragma unroll
+    for (int h = 0; h < 4; h++) {
+        uint32_t pos = valid ? bloom_pos(h, (uint32_t)id, nbits_mask) : 0u;
+        word[h] = pos & wmask;  // word = pos % BF_WORDS
+        bitn[h] = pos >> wshift; // bit  = pos / BF_WORDS
+        pre[h] = valid ? bloom[word[h]] : 0ull;
+        miss[h] = !((pre[h] >> bitn[h]) & 1ull);
+        all = all && !miss[h];
+    }
+    const bool fresh = valid && !all;
+    bool dup = false;
+    if (fresh) {
+#pragma unroll
+        for (int h = 0; h < 4; h++) {
+            if (miss[h]) {
+                unsigned long long bit = 1ull << bitn[h];
+                unsigned long long old = atomicOr(&bloom[word[h]], bit);
+                dup = dup || ((old & bit) != 0ull);
+            }
+        }
+    }
+    const uint64_t fresh_mask = __ballot(fresh);
+    if (__ballot(dup) == 0ull) return fresh_mask;
+
+    // Two fresh slots share a position (or one slot hashes twice to it): put
+    // the words back as they were before this row and replay in slot order.
+    if (fresh) {
+#pragma unroll
+        for (int h = 0; h < 4; h++)
+            if (miss[h]) bloom[word[h]] = pre[h];
+    }
+    wave_lds_sync();
+    uint64_t acc = 0ull;
+    uint64_t fm = fresh_mask;
+    const int lane = lane_id();
+    while (fm) {
+        const int j = ffs64(fm);
+        fm &= fm - 1ull;
+        bool take = false;
+        if (lane == j) {
+            bool hit = true;
+#pragma unroll
+            for (int h = 0; h < 4; h++) hit = hit && ((bloom[word[h]] >> bitn[h]) & 1ull);
+            if (!hit) {
+#pragma unroll
+                for (int h = 0; h < 4; h++) bloom[word[h]] |= (1ull << bitn[h]);
+            }
+            take = !hit;
+        }
+        wave_lds_sync();
+        if (__ballot(take)) acc |= 1ull << j;
+    }
+    return acc;
+}
+
+// ---------------------------------------------------------------- the two sets
+
+// Unsorted bounded set of (dist, id), entry i in slot i/64 of lane i%64.
+template <int S>
+struct WSet {
+    float d[S];
+    int id[S];
+    int sz;  // wave-uniform
+    int cap; // wave-uniform
+
+    __device__ __forceinline__ void init(int c) {
+        sz = 0;
+        cap = c < 1 ? 1 : c;
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+            d[s] = 0.f;
+            id[s] = -1;
+        }
+    }
+    __device__ __forceinline__ float vmax(int lane) const {
+        float m = -INFINITY;
+#pragma unroll
+        for (int s = 0; s < S; s++)
+            if (s * 64 + lane < sz) m = fmaxf(m, d[s]);
+        return wave_max(m);
+    }
+    __device__ __forceinline__ float vmin(int lane) const {
+        float m = INFINITY;
+#pragma unroll
+        for (int s = 0; s < S; s++)
+            if (s * 64 + lane < sz) m = fminf(m, d[s]);
+        return wave_min(m);
+    }
+    // First index whose value equals v (v is a value present in the set).
+    __device__ __forceinline__ int first_eq(float v, int lane) const {
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+            uint64_t b = __ballot(s * 64 + lane < sz && d[s] == v);
+            if (b) return s * 64 + ffs64(b);
+        }
+        return 0;
+    }
+    __device__ __forceinline__ void get(int idx, float& dv, int& iv) const {
+        const int slot = idx >> 6, ln = idx & 63;
+        float dd = d[0];
+        int ii = id[0];
+#pragma unroll
+        for (int s = 1; s < S; s++)
+            if (slot == s) {
+                dd = d[s];
+                ii = id[s];
+            }
+        dv = readlane_f(dd, ln);
+        iv = __builtin_amdgcn_readlane(ii, ln);
+    }
+    __device__ __forceinline__ void put(int idx, float dv, int iv, int lane) {
+        if (lane == (idx & 63)) {
+            const int slot = idx >> 6;
+#pragma unroll
+            for (int s = 0; s < S; s++)
+                if (slot == s) {
+                    d[s] = dv;
+                    id[s] = iv;
+                }
+        }
+    }
+    // Remove entry p, filling the hole with the last entry (gpuHeap.h:44-48).
+    __device__ __forceinline__ void remove_at(int p, int lane) {
+        float ld;
+        int li;
+        get(sz - 1, ld, li);
+        put(p, ld, li, lane);
+        sz--;
+    }
+    // Push items it_d/it_id held by lanes 0..F-1, in lane order, with the
+    // Candidates::push / Results::try_push rule (gpuHeap.h:23-36, 78-87).
+    // Returns how many were accepted.
+    __device__ __forceinline__ int push_items(float it_d, int it_id, int F, int lane) {
+        int accepted = 0;
+        const int a = min(F, cap - sz);
+        if (a > 0) {
+            const int src = (lane - sz) & 63;
+            const float md = __shfl(it_d, src, 64);
+            const int mi = __shfl(it_id, src, 64);
+            if (src < a) {
+                const int slot = (sz + src) >> 6;
+#pragma unroll
+                for (int s = 0; s < S; s++)
+                    if (slot == s) {
+                        d[s] = md;
+                        id[s] = mi;
+                    }
+            }
+            sz += a;
+            accepted = a;
+        }
+        if (F > a) {
+            float M = vmax(lane);
+            int start = a;
+            for (;;) {
+                const uint64_t m = __ballot(lane >= start && lane < F && it_d < M);
+                if (m == 0ull) break;
+                const int r = ffs64(m);
+                const float dv = readlane_f(it_d, r);
+                const int iv = __builtin_amdgcn_readlane(it_id, r);
+                const int p = first_eq(M, lane);
+                put(p, dv, iv, lane);
+                accepted++;
+                start = r + 1;
+                if (start >= F) break;
+                M = vmax(lane);
+            }
+        }
+        return accepted;
+    }
+};
+
+// ---------------------------------------------------------------- distances
+
+// Distances of the rows stage_id[0..F) to the query; lane 0 of each group of
+// G lanes writes stage_d[r].  Lane t of a group accumulates the 16-byte
+// chunks c = t, t+G, ... (c < dstride/4) of the row in order, with explicit
+// fmaf; the group sum is the xor butterfly of group_sum.
+template <int METRIC, int G, int QCH, int NP>
+__device__ __forceinline__ void float_distances(const float4* __restrict__ data, int d4,
+                                                const float4 (&q)[QCH], int F,
+                                                const int* stage_id, float* stage_d, int lane) {
+    constexpr int NG = 64 / G;
+    const int grp = lane / G, t = lane % G;
+    const int passes = (F + NG - 1) / NG;
+    for (int p0 = 0; p0 < passes; p0 += NP) {
+        float4 x[NP][QCH];
+        int rr[NP];
+#pragma unroll
+        for (int pp = 0; pp < NP; pp++) {
+            const int r = (p0 + pp) * NG + grp;
+            rr[pp] = r;
+            const int node = r < F ? stage_id[r] : -1;
+            const float4* row = data + (size_t)(node < 0 ? 0 : node) * d4;
+#pragma unroll
+            for (int i = 0; i < QCH; i++) {
+                const int c = t + i * G;
+                if (node >= 0 && c < d4) x[pp][i] = row[c];
+                else x[pp][i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+        }
+#pragma unroll
+        for (int pp = 0; pp < NP; pp++) {
+            float acc = 0.f;
+#pragma unroll
+            for (int i = 0; i < QCH; i++) {
+                const int c = t + i * G;
+                if (c < d4) {
+                    if constexpr (METRIC == METRIC_IP) {
+                        acc = __builtin_fmaf(q[i].x, x[pp][i].x, acc);
+                        acc = __builtin_fmaf(q[i].y, x[pp][i].y, acc);
+                        acc = __builtin_fmaf(q[i].z, x[pp][i].z, acc);
+                        acc = __builtin_fmaf(q[i].w, x[pp][i].w, acc);
+                    } else {
+                        float e;
+                        e = q[i].x - x[pp][i].x;
+                        acc = __builtin_fmaf(e, e, acc);
+                        e = q[i].y - x[pp][i].y;
+                        acc = __builtin_fmaf(e, e, acc);
+                        e = q[i].z - x[pp][i].z;
+                        acc = __builtin_fmaf(e, e, acc);
+                        e = q[i].w - x[pp][i].w;
+                        acc = __builtin_fmaf(e, e, acc);
+                    }
+                }
+            }
+            acc = group_sum<G>(acc);
+            if (t == 0 && rr[pp] < F) stage_d[rr[pp]] = METRIC == METRIC_IP ? -acc : acc;
+        }
+    }
+}
+
+// Hamming on 256-bit codes (4 x u64): two lanes per code, 16 bytes each
+// (warp_hamming, hashKernelGraph.h:215-224).
+__device__ __forceinline__ void hamming_distances(const ulonglong2* __restrict__ codes, ulonglong2 q,
+                                                  int F, const int* stage_id, float* stage_d, int lane) {
+    const int grp = lane >> 1, t = lane & 1;
+    for (int p = 0; p * 32 < F; p++) {
+        const int r = p * 32 + grp;
+        const int node = r < F ? stage_id[r] : -1;
+        ulonglong2 x = make_ulonglong2(0ull, 0ull);
+        if (node >= 0) x = codes[(size_t)node * 2 + t];
+        int c = __popcll(q.x ^ x.x) + __popcll(q.y ^ x.y);
+        c = c + dppi<DPP_XOR1>(c);
+        if (t == 0 && r < F) stage_d[r] = (float)c;
+    }
+}
+
+// ---------------------------------------------------------------- the kernel
+
+template <int METRIC, int S, int G, int QCH>
+__global__ void __launch_bounds__(256) search_kernel(SearchArgs P) {
+    extern __shared__ unsigned long long smem[];
+    const int lane = lane_id();
+    const int wave = threadIdx.x >> 6;
+    const int qid = blockIdx.x * (blockDim.x >> 6) + wave;
+    if (qid >= P.nq) return;
+
+    // LDS per wave: bloom[W] | stage_id[64] | stage_d[64]
+    unsigned long long* bloom = smem + (size_t)wave * (P.bloom_words + 64);
+    int* stage_id = (int*)(bloom + P.bloom_words);
+    float* stage_d = (float*)(stage_id + 64);
+    const uint32_t wmask = (uint32_t)P.bloom_words - 1u;
+    const int wshift = P.bloom_shift;
+    const uint32_t nbits_mask = (uint32_t)P.bloom_words * 64u - 1u;
+
+    for (int w = lane; w < P.bloom_words; w += 64) bloom[w] = 0ull;
+
+    // query in registers (each lane holds the chunks it multiplies)
+    float4 qf[QCH];
+    ulonglong2 qc = make_ulonglong2(0ull, 0ull);
+    const int d4 = P.dstride >> 2;
+    if constexpr (METRIC == METRIC_HAMMING) {
+        qc = ((const ulonglong2*)P.queries)[(size_t)qid * 2 + (lane & 1)];
+    } else {
+        const float4* qrow = (const float4*)P.queries + (size_t)qid * d4;
+        const int t = lane % G;
+#pragma unroll
+        for (int i = 0; i < QCH; i++) {
+            const int c = t + i * G;
+            qf[i] = c < d4 ? qrow[c] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+    }
+    wave_lds_sync();
+
+    WSet<S> cand, res;
+    cand.init(P.ef);
+    res.init(P.ef);
+    // QueryTrace, telemetry.h:5-18 (wave-uniform counters)
+    int tr_seed = 0, tr_visited = 0, tr_expanded = 0, tr_slots = 0, tr_cand = 0, tr_devals = 0;
+    int tr_fpush = 0, tr_rupd = 0, tr_peakf = 0, tr_peakr = 0, tr_early = 0;
+
+    auto distances = [&](int F) {
+        if constexpr (METRIC == METRIC_HAMMING) {
+            hamming_distances((const ulonglong2*)P.data, qc, F, stage_id, stage_d, lane);
+        } else {
+            float_distances<METRIC, G, QCH, 2>((const float4*)P.data, d4, qf, F, stage_id, stage_d, lane);
+        }
+    };
+
+    // Bloom-test a row of ids (lane order), compute the distances of the new
+    // ones and push them to both sets in order.  Returns the number of new ids.
+    auto visit_row = [&](int id, bool valid) -> int {
+        const uint64_t acc = bloom_row(bloom, wmask, wshift, nbits_mask, id, valid);
+        const int F = __popcll(acc);
+        if (F == 0) return 0;
+        const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(acc >> 32),
+                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)acc, 0u));
+        if ((acc >> lane) & 1ull) stage_id[rank] = id;
+        wave_lds_sync();
+        distances(F);
+        wave_lds_sync();
+        const float it_d = lane < F ? stage_d[lane] : INFINITY;
+        const int it_id = lane < F ? stage_id[lane] : -1;
+        tr_fpush += cand.push_items(it_d, it_id, F, lane);
+        tr_rupd += res.push_items(it_d, it_id, F, lane);
+        tr_peakf = max(tr_peakf, cand.sz);
+        tr_peakr = max(tr_peakr, res.sz);
+        wave_lds_sync();
+        return F;
+    };
+
+    // Seeds: kernalGraph.h:215-231 -- ids (size_t)p * n / n_seeds, or the
+    // caller's entry list.
+    {
+        int sid = 0;
+        bool sv = lane < P.n_entry;
+        if (sv) {
+            if (P.entries) {
+                sid = P.entries[(size_t)qid * P.n_entry + lane];
+                sv = sid >= 0 && sid < P.n;
+            } else {
+                sid = (int)(((uint64_t)lane * (uint64_t)P.n) / (uint64_t)P.n_entry);
+            }
+        }
+        const int F = visit_row(sid, sv);
+        tr_seed += F;
+        tr_visited += F;
+        tr_devals += F;
+    }
+
+    // Main loop: kernalGraph.h:233-289.
+    for (;;) {
+        if (cand.sz == 0) break;
+        const float c_min = cand.vmin(lane);
+        const int p = cand.first_eq(c_min, lane);
+        float dummy;
+        int cn;
+        cand.get(p, dummy, cn);
+        cand.remove_at(p, lane);
+        tr_expanded++;
+        if (res.sz == res.cap) {
+            const float worst = res.vmax(lane);
+            if (c_min > worst) {
+                tr_early = 1;
+                break;
+            }
+        }
+        tr_slots += P.block;
+        const int nb = lane < P.block ? P.adj[(size_t)cn * P.block + lane] : -1;
+        const int F = visit_row(nb, nb >= 0);
+        tr_visited += F;
+        tr_cand += F;
+        tr_devals += F;
+    }
+
+    // Trim to k, then drain best-first (kernalGraph.h:292-298, gpuHeap.h:90-108).
+    while (res.sz > P.k) {
+        const float M = res.vmax(lane);
+        res.remove_at(res.first_eq(M, lane), lane);
+    }
+    const int cnt = res.sz < P.k ? res.sz : P.k;
+    int32_t* out = P.ids_out + (size_t)qid * P.k;
+    float* dout = P.dists_out ? P.dists_out + (size_t)qid * P.k : nullptr;
+    for (int i = 0; i < cnt; i++) {
+        const float m = res.vmin(lane);
+        const int p = res.first_eq(m, lane);
+        float dv;
+        int iv;
+        res.get(p, dv, iv);
+        res.remove_at(p, lane);
+        if (lane == 0) {
+            out[i] = iv;
+            if (dout) dout[i] = dv;
+        }
+    }
+    for (int i = cnt + lane; i < P.k; i += 64) {
+        out[i] = -1;
+        if (dout) dout[i] = INFINITY;
+    }
+    if (P.trace_out && lane == 0) {
+        int32_t* t = P.trace_out + (size_t)qid * 12;
+        t[0] = tr_seed;
+        t[1] = tr_visited;
+        t[2] = tr_expanded;
+        t[3] = tr_slots;
+        t[4] = tr_cand;
+        t[5] = tr_devals;
+        t[6] = tr_fpush;
+        t[7] = tr_rupd;
+        t[8] = tr_peakf;
+        t[9] = tr_peakr;
+        t[10] = cnt;
+        t[11] = tr_early;
+    }
+}
+
+// ---------------------------------------------------------------- dispatch
+
+template <int METRIC, int S, int G, int QCH>
+static hipError_t launch_one(const SearchArgs& a, int waves_per_block, hipStream_t st) {
+    const int threads = 64 * waves_per_block;
+    const int blocks = (a.nq + waves_per_block - 1) / waves_per_block;
+    const size_t lds = (size_t)waves_per_block * ((size_t)a.bloom_words + 64) * 8;
+    hipLaunchKernelGGL((search_kernel<METRIC, S, G, QCH>), dim3(blocks), dim3(threads), lds, st, a);
+    return hipGetLastError();
+}
+
+template <int METRIC, int S>
+static hipError_t launch_g(const SearchArgs& a, int G, int qch, int wpb, hipStream_t st) {
+    if constexpr (METRIC == METRIC_HAMMING) {
+        return launch_one<METRIC, S, 2, 1>(a, wpb, st);
+    } else {
+        if (qch == 4) {
+            switch (G) {
+            case 2: return launch_one<METRIC, S, 2, 4>(a, wpb, st);
+            case 4: return launch_one<METRIC, S, 4, 4>(a, wpb, st);
+            case 8: return launch_one<METRIC, S, 8, 4>(a, wpb, st);
+            case 16: return launch_one<METRIC, S, 16, 4>(a, wpb, st);
+            case 32: return launch_one<METRIC, S, 32, 4>(a, wpb, st);
+            case 64: return launch_one<METRIC, S, 64, 4>(a, wpb, st);
+            }
+        } else if (G == 64 && qch == 8) {
+            return launch_one<METRIC, S, 64, 8>(a, wpb, st);
+        } else if (G == 64 && qch == 16) {
+            return launch_one<METRIC, S, 64, 16>(a, wpb, st);
+        }
+        return hipErrorInvalidValue;
+    }
+}
+
+template <int METRIC>
+static hipError_t launch_s(const SearchArgs& a, int S, int G, int qch, int wpb, hipStream_t st) {
+    switch (S) {
+    case 1: return launch_g<METRIC, 1>(a, G, qch, wpb, st);
+    case 2: return launch_g<METRIC, 2>(a, G, qch, wpb, st);
+    case 4: return launch_g<METRIC, 4>(a, G, qch, wpb, st);
+    case 8: return launch_g<METRIC, 8>(a, G, qch, wpb, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_search(const SearchArgs& a, int metric, int S, int G, int qch, int wpb, hipStream_t st) {
+    switch (metric) {
+    case METRIC_L2: return launch_s<METRIC_L2>(a, S, G, qch, wpb, st);
+    case METRIC_IP: return launch_s<METRIC_IP>(a, S, G, qch, wpb, st);
+    case METRIC_HAMMING: return launch_s<METRIC_HAMMING>(a, S, G, qch, wpb, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace song

@@ -1,0 +1,227 @@
+"""Benchmark: plain wave64 SONG search on one MI355X, 1M x 128 L2 workload.
+
+Workload: 1,000,000 x 128 float32 vectors, integer values in [0, 255] from a
+1,000-centre Gaussian mixture (sigma 25), 10,000 held-out queries, NSW graph
+with out-degree 32 (batched CPU build), L2, k = 10.  Queue sizes 32..512 are
+swept (k in {10, 100}); the timed queue is the smallest reaching
+recall@10 >= 0.90 against GPU brute-force ground truth.
+
+A "step" is one search of all 10,000 queries with queries, graph and vectors
+already resident in HBM (song_search_device: one kernel launch).  value =
+queries per second over the timed steps.  The JSON line also carries the
+roofline of the search kernel (algorithmic bytes from the run's own trace
+counters / HIP-event kernel time) and the reference's CPU search
+(Graph::search, graph.h:33-115, compiled from its sources into oracle/_ref)
+timed on this host on a bounded sample.
+"""
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def gpu_ground_truth(base, queries, k):
+    """Exact k-NN by brute force on the GPU (fp32 GEMM; exact for integer data, dim <= 258)."""
+    import torch
+
+    dev = torch.device("cuda:0")
+    b = torch.from_numpy(base).to(dev)
+    q = torch.from_numpy(queries).to(dev)
+    bn = (b * b).sum(1)
+    qn = (q * q).sum(1)
+    best_d = None
+    best_i = None
+    step = 1 << 17
+    for s in range(0, b.shape[0], step):
+        bb = b[s:s + step]
+        d = qn[:, None] + bn[None, s:s + step] - 2.0 * (q @ bb.T)
+        dv, di = torch.topk(d, k, dim=1, largest=False)
+        di = di + s
+        if best_d is None:
+            best_d, best_i = dv, di
+        else:
+            cd = torch.cat([best_d, dv], 1)
+            ci = torch.cat([best_i, di], 1)
+            best_d, sel = torch.topk(cd, k, dim=1, largest=False)
+            best_i = torch.gather(ci, 1, sel)
+    out = best_i.int().cpu().numpy()
+    del b, q
+    torch.cuda.empty_cache()
+    return out
+
+
+def recall_at(ids, gt, k):
+    hits = 0
+    for a, g in zip(ids[:, :k], gt[:, :k]):
+        hits += len(set(a.tolist()) & set(g.tolist()))
+    return hits / (ids.shape[0] * k)
+
+
+def algorithmic_bytes(trace, dim, block, k):
+    """Sum over queries of distance_evals*row + expanded*row_of_adjacency + query + ids."""
+    devals = trace[:, 5].astype(np.int64).sum()
+    expanded = trace[:, 2].astype(np.int64).sum()
+    nq = trace.shape[0]
+    return int(devals * 4 * dim + expanded * 4 * block + nq * (4 * dim + 4 * k))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--nq", type=int, default=10_000)
+    ap.add_argument("--k", type=int, default=10)
+    ap.add_argument("--ef", type=int, default=0, help="timed queue size (0: smallest with recall >= target)")
+    ap.add_argument("--target-recall", type=float, default=0.90)
+    ap.add_argument("--threads", type=int, default=0)
+    ap.add_argument("--graph", default="", help="load/save the graph file here")
+    ap.add_argument("--cpu-sample", type=int, default=4000, help="queries timed on the CPU baseline")
+    ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(multi-GPU sharding is a design note only)"}))
+        sys.exit(2)
+
+    song = importlib.import_module("2020_song_ex_amd")
+    datasets = importlib.import_module("2020_song_ex_amd.datasets")
+    N = song._native
+    threads = args.threads or min(16, os.cpu_count() or 1)
+
+    t0 = time.time()
+    cfg = datasets.config2(n=args.n, nq=args.nq)
+    base, queries, block = cfg["base"], cfg["queries"], cfg["block"]
+    dim = base.shape[1]
+    log(f"data {base.shape} queries {queries.shape} generated in {time.time() - t0:.1f}s")
+
+    adj = None
+    if args.graph and os.path.exists(args.graph):
+        adj, _ = song.graph_load(args.graph)
+        log(f"graph loaded from {args.graph}")
+    if adj is None:
+        t0 = time.time()
+        adj = song.nsw_build(base, block=block, n_threads=threads, batch_div=64, seq_prefix=1024,
+                             exact_order=False, verbose=True)
+        log(f"NSW graph built in {time.time() - t0:.1f}s ({threads} threads, batched)")
+        if args.graph:
+            song.graph_save(args.graph, adj, dim=dim)
+
+    idx = song.Index(base, adj)
+    dq = idx.queries_to_device(queries)
+    nq = queries.shape[0]
+    kmax = 100
+    d_ids = N.DeviceBuffer(nq * kmax * 4)
+    d_tr = N.DeviceBuffer(nq * 48)
+
+    t0 = time.time()
+    gt = gpu_ground_truth(base, queries, kmax)
+    log(f"brute-force ground truth in {time.time() - t0:.1f}s")
+
+    def run(k, ef, trace=True):
+        tm = idx.search_device(dq, nq, k, ef, d_ids, None, d_tr if trace else None)
+        ids = d_ids.download(np.empty((nq, k), np.int32))
+        tr = d_tr.download(np.empty((nq, 12), np.int32)) if trace else None
+        return ids, tr, tm
+
+    sweep = []
+    efs = (32, 64, 128, 256, 512)
+    if not args.no_sweep:
+        for k in (10, 100):
+            for ef in efs:
+                if ef < k:
+                    continue
+                ids, tr, tm = run(k, ef)
+                rec = recall_at(ids, gt, k)
+                rec10 = recall_at(ids, gt, 10) if k >= 10 else None
+                row = {"k": k, "ef": ef, "recall": round(rec, 4), "recall_at_10": rec10,
+                       "kernel_ms": round(tm["kernel_ms"], 3), "qps_kernel": round(nq / (tm["kernel_ms"] / 1e3)),
+                       "evals_mean": float(tr[:, 5].mean()), "expanded_mean": float(tr[:, 2].mean())}
+                sweep.append(row)
+                log(f"sweep {row}")
+
+    ef = args.ef
+    if not ef:
+        ef = efs[-1]
+        for e in efs:
+            ids, _, _ = run(args.k, e, trace=False)
+            if recall_at(ids, gt, 10) >= args.target_recall:
+                ef = e
+                break
+    ids, trace, tm = run(args.k, ef)
+    rec = recall_at(ids, gt, 10)
+    log(f"timed queue size {ef}: recall@10 {rec:.4f}")
+
+    for _ in range(args.warmup):
+        run(args.k, ef, trace=False)
+    kernel_ms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        tm = idx.search_device(dq, nq, args.k, ef, d_ids, None, None)
+        kernel_ms += tm["kernel_ms"]
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1e3 / args.steps
+    qps = nq * args.steps / wall
+    kernel_avg_ms = kernel_ms / args.steps
+
+    abytes = algorithmic_bytes(trace, dim, block, args.k)
+    achieved = abytes / (kernel_avg_ms / 1e3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "search_kernel<L2,S=1,G=8,QCH=4>", "kernel_avg_ms": round(kernel_avg_ms, 4),
+                "algorithmic_bytes_per_launch": abytes}
+
+    cpu = None
+    if not args.no_cpu:
+        oracle = importlib.import_module("oracle")
+        sample = min(args.cpu_sample, nq)
+        if oracle.ref_available():
+            cids, _, cms, _ = oracle.ref_graph_search(base, adj, queries[:sample], args.k, ef, n_threads=1)
+            kind = "reference"
+            cids = cids[:, ::-1]
+        else:
+            t0 = time.perf_counter()
+            cids, _ = oracle.cpu_search(base, adj, queries[:sample], args.k, ef, block=block)
+            cms = (time.perf_counter() - t0) * 1e3
+            kind = "port"
+        cpu = {"value": round(sample / (cms / 1e3), 1), "unit": "queries/s", "cores": 1, "kind": kind,
+               "sample": f"{sample} queries of the same workload, queue {ef}, k {args.k}, 1 thread "
+                         f"(Graph::search, graph.h:33-115)",
+               "recall_at_10": round(recall_at(cids, gt[:sample], 10), 4)}
+        log(f"cpu baseline {cpu}")
+
+    out = {
+        "metric": "QPS at recall@10=0.90, 1M×128 L2: learned+persistent vs plain; p99 latency",
+        "value": round(qps, 1), "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"plain SONG search, {args.n}x{dim} integer GMM (1000 centres, sigma 25), "
+                               f"{nq} queries, NSW out-degree {block}, L2",
+                   "method": "plain", "k": args.k, "queue": ef, "recall_at_10": round(rec, 4),
+                   "kernel_qps": round(nq / (kernel_avg_ms / 1e3), 1)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "sweep": sweep,
+    }
+    print(json.dumps(out), flush=True)
+    idx.close()
+
+
+if __name__ == "__main__":
+    main()

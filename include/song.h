@@ -1,0 +1,217 @@
+/*
+ * song.h -- C ABI of libsong.so, the MI355X-native SONG graph search.
+ *
+ * The reference exposes its search as a C++ header API (KernelGraph /
+ * HashKernelGraph, build_kernel_graph, upload_to_gpu, gpu_search) that its
+ * bench_gpu driver calls in-process, and run_compare.py drives bench_gpu as a
+ * subprocess.  These entry points replace that header API one for one; each
+ * declaration cites the reference interface it replaces.  No C++ types or
+ * exceptions cross the boundary: plain pointers, sizes and int status codes.
+ *
+ * Conventions
+ *   - Every call returns 0 (SONG_OK) or a negative SONG_ERR_* code;
+ *     song_last_error() then holds a message (thread-local).
+ *   - The caller owns every host buffer.  The library owns the device copy
+ *     of the index (song_index_create / song_index_destroy).
+ *   - One song_index is used by one host thread at a time; its work is
+ *     serialised on the index's own HIP stream.
+ *   - Out-of-range arguments are errors, never silently clamped.  The one
+ *     rule kept from the reference is ef := max(ef, k) (kernalGraph.h:315);
+ *     its ef <= 50 cap (kernalGraph.h:316) is lifted to ef <= 512.
+ *   - Metrics: L2 = squared Euclidean; IP = inner product stored as -<q,x>
+ *     (smaller is better everywhere); HAMMING = popcount distance on 256-bit
+ *     codes stored as 4 x uint64 (the bits of the reference's 8 x uint32).
+ */
+#ifndef SONG_H
+#define SONG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SONG_ABI_VERSION 1
+
+enum {
+    SONG_METRIC_L2 = 0,
+    SONG_METRIC_IP = 1,
+    SONG_METRIC_HAMMING = 2
+};
+
+enum {
+    SONG_OK = 0,
+    SONG_ERR_ARG = -1,   /* an argument is out of range or null */
+    SONG_ERR_HIP = -2,   /* the HIP runtime reported an error */
+    SONG_ERR_ALLOC = -3, /* a device or host allocation failed */
+    SONG_ERR_IO = -4,    /* a file could not be read or written */
+    SONG_ERR_STATE = -5  /* the call does not fit the object's state */
+};
+
+typedef struct song_index song_index;
+
+/* Per-query counters, field for field QueryTrace (telemetry.h:5-18). */
+typedef struct song_query_trace {
+    int32_t seed_count;
+    int32_t visited_nodes;
+    int32_t expanded_nodes;
+    int32_t neighbor_slots_scanned;
+    int32_t candidate_nodes;
+    int32_t distance_evals;
+    int32_t frontier_pushes;
+    int32_t result_updates;
+    int32_t peak_frontier;
+    int32_t peak_results;
+    int32_t result_count;
+    int32_t early_stopped;
+} song_query_trace;
+
+/* Timing of one search call, the fields of GpuSearchTrace (telemetry.h:30-41)
+ * without the per-batch and per-query vectors.  h2d/kernel/d2h are measured
+ * with HIP events on the index stream; total_ms is host wall time of the
+ * call's batch loop (allocation excluded), as gpu_search's total_ms
+ * (kernalGraph.h:341,376). */
+typedef struct song_timing {
+    double alloc_ms;
+    double h2d_ms;
+    double memset_ms;
+    double kernel_ms;
+    double d2h_ms;
+    double total_ms;
+    int32_t n_batches;
+    int32_t ef_used;
+} song_timing;
+
+/* Per-batch timing, BatchTiming (telemetry.h:20-28). */
+typedef struct song_batch_timing {
+    int32_t batch_index;
+    int32_t batch_offset;
+    int32_t batch_size;
+    int32_t pad_;
+    double h2d_ms;
+    double memset_ms;
+    double kernel_ms;
+    double d2h_ms;
+} song_batch_timing;
+
+/* Search options; zero-initialise and set what you need.  All knobs are
+ * compile-time constants in the reference (config.h:9-58). */
+typedef struct song_search_opts {
+    int32_t bloom_bits;      /* visited-set bits, power of two in [2^9, 2^16];
+                                0 = 16384 (BF_WORDS*64, config.h:56) */
+    int32_t batch;           /* queries per kernel launch; 0 = all at once
+                                (N_MULTIQUERY, config.h:38, is 512) */
+    int32_t n_seeds;         /* evenly spaced entry points, 1..64;
+                                0 = 4 (N_MULTIPROBE, config.h:34) */
+    int32_t waves_per_block; /* 1, 2 or 4 queries per workgroup; 0 = 4 */
+    int32_t lanes_per_row;   /* lanes that share one vector row (power of
+                                two, 2..64); 0 = chosen from dim.  Fixes the
+                                float summation order (see DESIGN). */
+    int32_t reserved[11];
+} song_search_opts;
+
+/* Version of this header's ABI (SONG_ABI_VERSION). */
+int song_abi_version(void);
+
+/* Message of the last error on this thread ("" if none). */
+const char* song_last_error(void);
+
+/* Number of visible HIP devices. */
+int song_device_count(int32_t* out);
+
+/* ---------------------------------------------------------------- build */
+
+/* CPU NSW build.  Replaces build_kernel_graph (kernalGraph.h:140-149,
+ * = build_graph, graph.h:190-199) and build_hashed_graph's graph part
+ * (hashKernelGraph.h:196-207); parameters are config.h's BLOCK_SIZE,
+ * FIXED_DEGREE, SEARCH_DEGREE and EF_CONSTRUCTION (config.h:9-24).
+ *   vecs: f32 [n x dim] (L2, IP) or uint64 [n x 4] (HAMMING, dim = 256)
+ *   batch_div: 0 = the reference's exact sequential insertion;
+ *              D > 0 = batched insertion, batch size s / D after the first
+ *              seq_prefix nodes (deterministic for any n_threads)
+ *   exact_order: 1 = distance.h:5-12 summation order, 0 = 8 partial sums
+ *   adj_out: [n x block], caller-owned; -1 marks empty slots. */
+int song_nsw_build(const void* vecs, int64_t n, int32_t dim, int32_t metric, int32_t block,
+                   int32_t fixed_deg, int32_t search_deg, int32_t ef_construction,
+                   int32_t n_threads, int32_t batch_div, int64_t seq_prefix, int32_t exact_order,
+                   int32_t verbose, int32_t* adj_out);
+
+/* Graph file (no counterpart in the reference, which rebuilds every run,
+ * generate_report.py:625-627): header + int32 adjacency. */
+int song_graph_save(const char* path, const int32_t* adj, int64_t n, int32_t block, int32_t metric,
+                    int32_t dim);
+int song_graph_load_header(const char* path, int64_t* n, int32_t* block, int32_t* metric,
+                           int32_t* dim);
+int song_graph_load(const char* path, int32_t* adj_out, int64_t n, int32_t block);
+
+/* 1-bit Gaussian random projection (BitHash, bitHash.h:17-56): the
+ * [256 x dim] matrix from std::mt19937(seed) + normal_distribution<float>
+ * (the reference's query/base hashing uses seed 42, bitHash.h:22), and the
+ * projection of n vectors to 256-bit codes (BitHash::project, bitHash.h:32-44;
+ * project_all, :50-55) as 4 x uint64 per vector. */
+int song_bithash_matrix(int32_t dim, uint32_t seed, float* out);
+int song_bithash_project(const float* R, int32_t dim, const float* vecs, int64_t n, int32_t n_threads,
+                         uint64_t* codes_out);
+
+/* ---------------------------------------------------------------- index */
+
+/* Device-resident index: vectors (or codes) and adjacency copied to HBM.
+ * Replaces the KernelGraph / HashKernelGraph constructors plus
+ * upload_to_gpu (kernalGraph.h:29-43, hashKernelGraph.h:31-33,93-98).
+ * Every adjacency entry must be -1 or a node id < n (checked).
+ * block <= 64.  dim: floats per vector (L2, IP) or 256 (HAMMING). */
+int song_index_create(const void* vecs, const int32_t* adj, int64_t n, int32_t dim, int32_t block,
+                      int32_t metric, int32_t device, song_index** out);
+
+/* Frees the device copy (~KernelGraph, kernalGraph.h:45). */
+void song_index_destroy(song_index* index);
+
+/* Floats (or uint64 words for HAMMING) per stored query row on the device:
+ * dim rounded up to a multiple of 4 (4 for HAMMING). */
+int song_index_row_stride(const song_index* index, int32_t* out);
+
+/* ---------------------------------------------------------------- search */
+
+/* Batched graph search of host queries.  Replaces
+ *   gpu_search(const KernelGraph&, const float* queries, nq, k,
+ *              N_MULTIQUERY, ef_search, GpuSearchTrace*)   kernalGraph.h:307-382
+ *   gpu_search(const HashKernelGraph&, const uint32_t* queries, ...)
+ *                                                       hashKernelGraph.h:360-435
+ * Per batch: queries H2D, one kernel launch, ids (+ dists, trace) D2H.
+ * Results are those of the reference kernel (kg_search_kernel,
+ * kernalGraph.h:184-299) for the same graph, seeds, ef and Bloom size.
+ *   queries: f32 [nq x dim] or uint64 [nq x 4]
+ *   ids_out: [nq x k], best first, -1 padded
+ *   dists_out, trace_out, timing_out, batch_out: nullable
+ *   batch_out: room for ceil(nq / batch) entries (BatchTiming list). */
+int song_search(song_index* index, const void* queries, int32_t nq, int32_t k, int32_t ef,
+                const song_search_opts* opts, int32_t* ids_out, float* dists_out,
+                song_query_trace* trace_out, song_timing* timing_out,
+                song_batch_timing* batch_out);
+
+/* The same search with every buffer already in device memory (queries laid
+ * out with song_index_row_stride floats per row); one launch per batch, no
+ * copies.  timing_out->kernel_ms is the HIP-event time of the launches on the
+ * index stream.  No counterpart in the reference (its gpu_search always
+ * copies); used to time the kernel with inputs resident in HBM. */
+int song_search_device(song_index* index, const void* d_queries, int32_t nq, int32_t k, int32_t ef,
+                       const song_search_opts* opts, int32_t* d_ids, float* d_dists,
+                       song_query_trace* d_trace, song_timing* timing_out);
+
+/* ---------------------------------------------------------------- device memory helpers */
+
+/* Plain device buffers for callers without a GPU runtime of their own
+ * (no counterpart in the reference, whose gpu_search allocates per call,
+ * kernalGraph.h:326-330). */
+int song_dev_alloc(int32_t device, int64_t bytes, void** out);
+int song_dev_free(void* p);
+int song_memcpy_h2d(void* dst, const void* src, int64_t bytes);
+int song_memcpy_d2h(void* dst, const void* src, int64_t bytes);
+/* Copies nq host queries into a new device buffer in the index's row layout. */
+int song_queries_to_device(song_index* index, const void* queries, int32_t nq, void** d_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SONG_H */

@@ -1,0 +1,188 @@
+"""GPU search vs the oracle, through the C ABI (libsong.so).
+
+Bar: ids, distances and all 12 QueryTrace counters identical to the oracle's
+re-execution of the reference GPU algorithm (kernalGraph.h:184-299) on the
+same graph, seeds, queue size and Bloom size.  The oracle sums float
+distances in the kernel's order, so the comparison is exact (tolerance 0)
+for float data too; the 1e-5-relative allowance of the spec is not needed.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def lanes_for(dim):
+    """The lane-group size the library picks for a dim (row chunks of 4 floats)."""
+    d4 = (dim + 3) // 4
+    g = 2
+    while g < 64 and g * 4 < d4:
+        g *= 2
+    return g
+
+
+def run_both(song, oracle, base, adj, queries, k, ef, metric="l2", bloom_bits=1 << 14, n_seeds=4,
+             lanes=0, waves_per_block=0, batch=0):
+    idx = song.Index(base, adj, metric=metric)
+    try:
+        r = idx.search(queries, k, ef, bloom_bits=bloom_bits, n_seeds=n_seeds, lanes_per_row=lanes,
+                       waves_per_block=waves_per_block, batch=batch)
+    finally:
+        idx.close()
+    g = lanes or (lanes_for(base.shape[1]) if metric != "hamming" else 2)
+    oi, od, ot = oracle.search(base, adj, queries, k, ef, block=adj.shape[1], metric=metric,
+                               bloom_bits=bloom_bits, n_seeds=n_seeds, group=g)
+    return r, (oi, od, ot)
+
+
+def assert_same(r, o, label=""):
+    oi, od, ot = o
+    bad = np.where((r.ids != oi).any(1))[0]
+    assert bad.size == 0, f"{label}: ids differ for queries {bad[:10]}: gpu {r.ids[bad[0]]} oracle {oi[bad[0]]}"
+    np.testing.assert_array_equal(r.dists.view(np.uint32), od.view(np.uint32), err_msg=f"{label}: dists")
+    badt = np.where((r.trace != ot).any(1))[0]
+    assert badt.size == 0, f"{label}: trace differs for {badt[:10]}: gpu {r.trace[badt[0]]} oracle {ot[badt[0]]}"
+
+
+@pytest.fixture(scope="module")
+def c1(song, datasets):
+    cfg = datasets.config1()
+    adj = song.nsw_build(cfg["base"], block=16, exact_order=False)
+    return cfg, adj
+
+
+def test_config1_exact(song, oracle, c1):
+    """Config 1: 10k x 128 integer GMM, out-degree 16, k=10, queue 64, Bloom 2^15."""
+    cfg, adj = c1
+    r, o = run_both(song, oracle, cfg["base"], adj, cfg["queries"], 10, 64, bloom_bits=1 << 15)
+    assert_same(r, o, "config1")
+    assert (r.trace[:, 0] == 4).all()
+
+
+@pytest.mark.parametrize("k,ef", [(1, 1), (10, 10), (10, 32), (10, 50), (10, 100), (10, 200),
+                                  (100, 100), (100, 256), (10, 512), (64, 64), (65, 65)])
+def test_queue_sizes(song, oracle, c1, k, ef):
+    cfg, adj = c1
+    r, o = run_both(song, oracle, cfg["base"], adj, cfg["queries"][:60], k, ef)
+    assert_same(r, o, f"k={k} ef={ef}")
+
+
+@pytest.mark.parametrize("bits", [512, 4096, 1 << 14, 1 << 16])
+def test_bloom_sizes(song, oracle, c1, bits):
+    """Small filters force false positives and in-row collisions."""
+    cfg, adj = c1
+    r, o = run_both(song, oracle, cfg["base"], adj, cfg["queries"][:60], 10, 64, bloom_bits=bits)
+    assert_same(r, o, f"bloom={bits}")
+
+
+@pytest.mark.parametrize("lanes", [2, 4, 8, 16, 32, 64])
+def test_lane_groups(song, oracle, datasets, lanes):
+    """Float data: every lane-group size gives the oracle's distances bit for bit."""
+    base, q = datasets.beta_mix(3000, 40, dim=100, n_centres=20, seed=3)
+    adj = song.nsw_build(base, block=32, exact_order=False)
+    r, o = run_both(song, oracle, base, adj, q, 10, 48, lanes=lanes)
+    assert_same(r, o, f"lanes={lanes}")
+
+
+@pytest.mark.parametrize("dim", [1, 3, 5, 31, 200, 960, 1500])
+def test_dims(song, oracle, dim):
+    rng = np.random.default_rng(dim)
+    base = rng.random((1500, dim), dtype=np.float32)
+    q = rng.random((30, dim), dtype=np.float32)
+    adj = song.nsw_build(base, block=32, exact_order=False)
+    r, o = run_both(song, oracle, base, adj, q, 10, 40)
+    assert_same(r, o, f"dim={dim}")
+
+
+def test_inner_product(song, oracle, datasets):
+    base, q = datasets.vmf_mix(4000, 50, dim=200, n_centres=50, kappa=30.0, seed=1)
+    adj = song.nsw_build(base, metric="ip", block=32, exact_order=False)
+    r, o = run_both(song, oracle, base, adj, q, 10, 64, metric="ip")
+    assert_same(r, o, "ip")
+
+
+def test_hamming(song, oracle, datasets):
+    """Hamming distances tie everywhere, so this checks the positional tie rules."""
+    vec, qv = datasets.vmf_mix(4000, 80, dim=64, n_centres=40, kappa=20.0, seed=2)
+    codes, planes = datasets.hyperplane_codes(vec, seed=5)
+    qcodes, _ = datasets.hyperplane_codes(qv, planes=planes)
+    adj = song.nsw_build(codes, metric="hamming", block=32)
+    for k, ef in [(10, 10), (10, 64), (10, 200)]:
+        r, o = run_both(song, oracle, codes, adj, qcodes, k, ef, metric="hamming")
+        assert_same(r, o, f"hamming k={k} ef={ef}")
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5])
+def test_tiny_graphs(song, oracle, n):
+    """n < 4: seeds repeat and are skipped by the Bloom filter; fewer than k results -> -1."""
+    rng = np.random.default_rng(n)
+    base = rng.integers(0, 9, size=(n, 8)).astype(np.float32)
+    adj = song.nsw_build(base, block=8)
+    q = rng.integers(0, 9, size=(5, 8)).astype(np.float32)
+    r, o = run_both(song, oracle, base, adj, q, 10, 16)
+    assert_same(r, o, f"n={n}")
+    assert (r.ids[:, n:] == -1).all()
+
+
+def test_isolated_nodes_and_seeds(song, oracle, c1):
+    """Rows emptied to -1 and 1..64 seeds."""
+    cfg, adj = c1
+    adj2 = adj.copy()
+    adj2[::7] = -1
+    for seeds in (1, 4, 17, 64):
+        r, o = run_both(song, oracle, cfg["base"], adj2, cfg["queries"][:40], 10, 32, n_seeds=seeds)
+        assert_same(r, o, f"seeds={seeds}")
+
+
+def test_batches_and_blocks(song, oracle, c1):
+    """Results do not depend on batch size or queries per workgroup."""
+    cfg, adj = c1
+    for wpb, batch in [(1, 0), (2, 7), (4, 33), (4, 512)]:
+        r, o = run_both(song, oracle, cfg["base"], adj, cfg["queries"], 10, 64,
+                        waves_per_block=wpb, batch=batch)
+        assert_same(r, o, f"wpb={wpb} batch={batch}")
+
+
+def test_duplicate_ids_in_row(song, oracle, c1):
+    """A row naming the same node twice: the second is a Bloom hit (in-row order)."""
+    cfg, adj = c1
+    adj2 = adj.copy()
+    adj2[:, 8:] = adj2[:, :8]
+    r, o = run_both(song, oracle, cfg["base"], adj2, cfg["queries"][:50], 10, 64)
+    assert_same(r, o, "dup rows")
+
+
+def test_empty_query_set(song, c1):
+    cfg, adj = c1
+    idx = song.Index(cfg["base"], adj)
+    r = idx.search(cfg["queries"][:0], 10, 64)
+    assert r.ids.shape == (0, 10)
+    idx.close()
+
+
+def test_device_resident_search_matches_host(song, c1):
+    cfg, adj = c1
+    idx = song.Index(cfg["base"], adj)
+    q = cfg["queries"]
+    host = idx.search(q, 10, 64)
+    dq = idx.queries_to_device(q)
+    N = song._native
+    d_ids = N.DeviceBuffer(q.shape[0] * 10 * 4)
+    d_tr = N.DeviceBuffer(q.shape[0] * 48)
+    tm = idx.search_device(dq, q.shape[0], 10, 64, d_ids, None, d_tr)
+    ids = d_ids.download(np.empty((q.shape[0], 10), np.int32))
+    tr = d_tr.download(np.empty((q.shape[0], 12), np.int32))
+    np.testing.assert_array_equal(ids, host.ids)
+    np.testing.assert_array_equal(tr, host.trace)
+    assert tm["kernel_ms"] > 0
+    idx.close()
+
+
+def test_recall_against_bruteforce(song, oracle, c1):
+    cfg, adj = c1
+    idx = song.Index(cfg["base"], adj)
+    r = idx.search(cfg["queries"], 10, 64, bloom_bits=1 << 15)
+    idx.close()
+    gt, _ = oracle.bruteforce(cfg["base"], cfg["queries"], 10)
+    rec = np.mean([len(set(a) & set(b)) / 10 for a, b in zip(r.ids, gt)])
+    assert rec > 0.9, rec
