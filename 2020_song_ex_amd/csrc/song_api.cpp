@@ -14,6 +14,7 @@
 
 #include "../../include/song.h"
 #include "nsw_build.h"
+#include "song_gt.h"
 #include "song_kernels.h"
 
 namespace {
@@ -464,6 +465,68 @@ int song_search_device(song_index* ix, const void* d_queries, int32_t nq, int32_
     tm.total_ms = ms_since(t0);
     tm.n_batches = bi;
     if (timing_out) *timing_out = tm;
+    return SONG_OK;
+}
+
+int song_index_bruteforce(song_index* ix, const void* queries, int32_t nq, int32_t k, int32_t* ids_out,
+                          float* dists_out, double* kernel_ms_out) {
+    if (!ix) return fail(SONG_ERR_ARG, "null index");
+    if (k < 1 || k > 128) return fail(SONG_ERR_ARG, "k must be in [1, 128]");
+    if (nq < 0 || (nq > 0 && (!queries || !ids_out))) return fail(SONG_ERR_ARG, "bad query buffers");
+    if (nq == 0) return SONG_OK;
+    std::lock_guard<std::mutex> lock(ix->mu);
+    HIP_TRY(hipSetDevice(ix->device), "hipSetDevice");
+    const size_t rb = row_bytes(ix);
+    int qw = ix->metric == SONG_METRIC_HAMMING ? 8 : ix->dstride <= 256 ? 8 : ix->dstride <= 1024 ? 4 : 2;
+    const int64_t groups = (nq + qw - 1) / qw;
+    int64_t parts = (16384 + groups - 1) / groups;
+    parts = std::max<int64_t>(1, std::min<int64_t>(parts, std::max<int64_t>(1, ix->n / 4096)));
+    parts = std::min<int64_t>(parts, 65535);
+    const int64_t slice = (ix->n + parts - 1) / parts;
+    void* dq = nullptr;
+    uint64_t* partial = nullptr;
+    int32_t* dids = nullptr;
+    float* dd = nullptr;
+    auto cleanup = [&]() {
+        if (dq) (void)hipFree(dq);
+        if (partial) (void)hipFree(partial);
+        if (dids) (void)hipFree(dids);
+        if (dd) (void)hipFree(dd);
+    };
+    hipError_t e = hipMalloc(&dq, (size_t)nq * rb);
+    if (e == hipSuccess) e = hipMalloc(&partial, (size_t)nq * parts * k * 8);
+    if (e == hipSuccess) e = hipMalloc(&dids, (size_t)nq * k * 4);
+    if (e == hipSuccess) e = hipMalloc(&dd, (size_t)nq * k * 4);
+    if (e != hipSuccess) {
+        cleanup();
+        return hip_fail(e, "hipMalloc (bruteforce workspace)");
+    }
+    const void* src = stage_rows(ix, queries, nq);
+    e = hipMemcpyAsync(dq, src, (size_t)nq * rb, hipMemcpyHostToDevice, ix->stream);
+    song::GtArgs a;
+    a.data = ix->d_data;
+    a.queries = dq;
+    a.partial = partial;
+    a.ids_out = dids;
+    a.dists_out = dd;
+    a.n = ix->n;
+    a.slice = slice;
+    a.dstride = ix->dstride;
+    a.nq = nq;
+    a.k = k;
+    a.parts = (int32_t)parts;
+    if (e == hipSuccess) e = hipEventRecord(ix->ev[1], ix->stream);
+    if (e == hipSuccess) e = song::launch_bruteforce(a, ix->metric, qw, ix->stream);
+    if (e == hipSuccess) e = hipEventRecord(ix->ev[2], ix->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(ids_out, dids, (size_t)nq * k * 4, hipMemcpyDeviceToHost, ix->stream);
+    if (e == hipSuccess && dists_out)
+        e = hipMemcpyAsync(dists_out, dd, (size_t)nq * k * 4, hipMemcpyDeviceToHost, ix->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ix->stream);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ix->ev[1], ix->ev[2]);
+    cleanup();
+    if (e != hipSuccess) return hip_fail(e, "bruteforce");
+    if (kernel_ms_out) *kernel_ms_out = ms;
     return SONG_OK;
 }
 

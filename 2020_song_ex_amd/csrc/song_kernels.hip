@@ -187,8 +187,12 @@ __device__ __forceinline__ uint64_t bloom_row(unsigned long long* bloom, uint32_
 // Unsorted bounded set of (dist, id), entry i in slot i/64 of lane i%64.
 template <int S>
 struct WSet {
-    float d[S];
-    int id[S];
+    // ext_vector_type keeps the slots in VGPRs: a plain array gets its
+    // per-slot selects folded back into an indexed scratch access.
+    typedef float fvec __attribute__((ext_vector_type(S == 1 ? 2 : S)));
+    typedef int ivec __attribute__((ext_vector_type(S == 1 ? 2 : S)));
+    fvec d;
+    ivec id;
     int sz;  // wave-uniform
     int cap; // wave-uniform
 

@@ -159,6 +159,17 @@ class Index:
                    for b in bt[: tm.n_batches]]
         return SearchResult(ids, d, tr, timing, batches)
 
+    def bruteforce(self, queries, k):
+        """Exact k-NN on the device over the resident vectors: (ids, dists, kernel_ms)."""
+        q = self._queries(queries)
+        nq = q.shape[0]
+        ids = np.empty((nq, k), np.int32)
+        d = np.empty((nq, k), np.float32)
+        ms = ctypes.c_double(0.0)
+        N.check(N.lib().song_index_bruteforce(self._h, N.ptr(q), nq, k, N.ptr(ids), N.ptr(d),
+                                              ctypes.byref(ms)), "song_index_bruteforce")
+        return ids, d, ms.value
+
     def queries_to_device(self, queries) -> N.DeviceBuffer:
         q = self._queries(queries)
         h = ctypes.c_void_p()

@@ -198,6 +198,14 @@ int song_search_device(song_index* index, const void* d_queries, int32_t nq, int
                        const song_search_opts* opts, int32_t* d_ids, float* d_dists,
                        song_query_trace* d_trace, song_timing* timing_out);
 
+/* Exact k nearest neighbours by brute force on the device, over the index's
+ * resident vectors (ground truth for recall).  Replaces the reference's CPU
+ * brute force (bench_cpu.cpp:158-175) and numpy compute_gt
+ * (run_compare.py:199-213).  Order: distance, then smaller id; k <= 128.
+ * kernel_ms_out (nullable): HIP-event time of the two kernels. */
+int song_index_bruteforce(song_index* index, const void* queries, int32_t nq, int32_t k,
+                          int32_t* ids_out, float* dists_out, double* kernel_ms_out);
+
 /* ---------------------------------------------------------------- device memory helpers */
 
 /* Plain device buffers for callers without a GPU runtime of their own

@@ -1,0 +1,66 @@
+"""The C ABI: every symbol include/song.h declares is exported by libsong.so,
+and argument errors come back as status codes (no GPU needed)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "song.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(song_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_symbols_exported(song):
+    lib = song._native.lib()
+    names = declared_functions()
+    assert len(names) >= 19
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(names) == set(song._native.EXPORTS)
+
+
+def test_abi_version(song):
+    assert song._native.lib().song_abi_version() == 1
+
+
+def test_oracle_libraries_load(oracle):
+    assert oracle.lib().or_bloom_pos(0, 0, 16384) >= 0
+
+
+def test_index_rejects_bad_adjacency(song):
+    """Checked on the host before any device call."""
+    data = np.zeros((10, 4), np.float32)
+    adj = np.full((10, 8), -1, np.int32)
+    adj[3, 2] = 10  # == n: out of range
+    with pytest.raises(song.SongError, match="adjacency"):
+        song.Index(data, adj)
+    adj[3, 2] = -2
+    with pytest.raises(song.SongError, match="adjacency"):
+        song.Index(data, adj)
+
+
+def test_index_rejects_bad_shapes(song):
+    with pytest.raises(song.SongError):
+        song.Index(np.zeros((10, 4), np.float32), np.full((10, 65), -1, np.int32))
+    with pytest.raises(song.SongError):
+        song.Index(np.zeros((10, 4097), np.float32), np.full((10, 8), -1, np.int32))
+
+
+def test_null_arguments(song):
+    L = song._native.lib()
+    assert L.song_index_row_stride(None, None) == -1
+    assert L.song_search(None, None, 0, 10, 10, None, None, None, None, None, None) == -1
+    assert b"null" in L.song_last_error()
+    assert L.song_graph_load_header(b"/nonexistent/graph.bin", ctypes.byref(ctypes.c_int64()),
+                                    ctypes.byref(ctypes.c_int32()), ctypes.byref(ctypes.c_int32()),
+                                    ctypes.byref(ctypes.c_int32())) == -4
+
+
+def test_device_count_does_not_fail(song):
+    assert song.device_count() >= 0

@@ -186,3 +186,29 @@ def test_recall_against_bruteforce(song, oracle, c1):
     gt, _ = oracle.bruteforce(cfg["base"], cfg["queries"], 10)
     rec = np.mean([len(set(a) & set(b)) / 10 for a, b in zip(r.ids, gt)])
     assert rec > 0.9, rec
+
+
+@pytest.mark.parametrize("metric,k", [("l2", 10), ("l2", 100), ("ip", 10), ("hamming", 10), ("l2", 128)])
+def test_gpu_bruteforce(song, oracle, datasets, metric, k):
+    """Ground truth on the device equals the oracle's brute force (ties -> smaller id)."""
+    if metric == "l2":
+        base, q = datasets.gmm_int(20000, 70, dim=128, n_centres=30, sigma=30.0, seed=k)
+    elif metric == "ip":
+        base, q = datasets.vmf_mix(8000, 40, dim=200, n_centres=20, kappa=20.0, seed=1)
+    else:
+        v, qv = datasets.vmf_mix(8000, 40, dim=64, n_centres=20, kappa=10.0, seed=2)
+        base, planes = datasets.hyperplane_codes(v, seed=3)
+        q, _ = datasets.hyperplane_codes(qv, planes=planes)
+    adj = np.full((base.shape[0], 8), -1, np.int32)
+    idx = song.Index(base, adj, metric=metric)
+    ids, d, ms = idx.bruteforce(q, k)
+    idx.close()
+    oi, od = oracle.bruteforce(base, q, k, metric=metric)
+    if metric == "ip":
+        # float data: the fp32 sums differ from the float64 oracle in the last bits
+        np.testing.assert_allclose(d, od, rtol=1e-5, atol=1e-5)
+        agree = np.mean([len(set(a) & set(b)) / k for a, b in zip(ids, oi)])
+        assert agree > 0.99
+    else:
+        np.testing.assert_array_equal(ids, oi)
+        np.testing.assert_array_equal(d, od.astype(np.float32))

@@ -1,0 +1,275 @@
+"""Pins of the oracle (oracle/song_oracle.c) against things it did not write.
+
+* The reference's own CPU code, compiled from its sources into oracle/_ref:
+  with the exact-visited, unbounded-frontier and sequential-sum knobs, the
+  oracle's GPU-algorithm re-execution must return exactly the reversed ids and
+  the same 12 trace counters as Graph::search (graph.h:33-115) on data with
+  distinct distances.  The oracle's Graph::search port must equal it always.
+* The Bloom hash (bloomFilter.h:23-30) restated independently in Python
+  integer arithmetic, the word/bit layout, and the false-positive rate against
+  (1 - exp(-4m/M))^4.
+* The two sets (gpuHeap.h): hand-derived known-answer sequences for the
+  first-index and swap-with-last rules, plus invariants on random sequences.
+* Distances: the sequential sum against the reference's own l2(); every
+  lane-group order against exact integer sums and float64.
+* Brute force against numpy.
+"""
+import numpy as np
+import pytest
+
+REF_ONLY = pytest.mark.skipif("not __import__('oracle').ref_available()",
+                              reason="oracle/_ref not built (needs the reference sources)")
+
+
+# ---------------------------------------------------------------- search vs Graph::search
+
+@REF_ONLY
+@pytest.mark.parametrize("ef,k", [(10, 10), (16, 5), (32, 10), (50, 10), (100, 10), (64, 1)])
+def test_knob_reduction_matches_reference_graph_search(oracle, ef, k):
+    rng = np.random.default_rng(ef * 7 + k)
+    data = rng.random((3000, 24), dtype=np.float32)
+    q = rng.random((60, 24), dtype=np.float32)
+    adj = oracle.ref_build_graph(data)
+    rid, rtr, _, _ = oracle.ref_graph_search(data, adj, q, k, ef)
+    oid, od, otr = oracle.search(data, adj, q, k, ef, block=32, ref_order=True, exact_visited=True,
+                                 unbounded=True)
+    # Graph::search returns worst-first; the GPU algorithm best-first.
+    cnt = (rid >= 0).sum(1)
+    for i in range(q.shape[0]):
+        np.testing.assert_array_equal(oid[i, :cnt[i]], rid[i, :cnt[i]][::-1])
+    np.testing.assert_array_equal(otr, rtr)
+
+
+@REF_ONLY
+@pytest.mark.parametrize("ef", [10, 30, 50])
+def test_cpu_port_matches_reference_with_ties(oracle, ef):
+    """Integer data (distance ties) -- the port's (d, id) heap order equals the reference's."""
+    rng = np.random.default_rng(ef)
+    data = rng.integers(0, 4, size=(2500, 8)).astype(np.float32)
+    q = rng.integers(0, 4, size=(50, 8)).astype(np.float32)
+    adj = oracle.ref_build_graph(data)
+    rid, rtr, _, _ = oracle.ref_graph_search(data, adj, q, 10, ef)
+    pid, ptr = oracle.cpu_search(data, adj, q, 10, ef, block=32)
+    np.testing.assert_array_equal(pid, rid)
+    np.testing.assert_array_equal(ptr, rtr)
+
+
+@REF_ONLY
+def test_reference_threads_do_not_change_results(oracle):
+    rng = np.random.default_rng(5)
+    data = rng.random((2000, 16), dtype=np.float32)
+    q = rng.random((64, 16), dtype=np.float32)
+    adj = oracle.ref_build_graph(data)
+    a = oracle.ref_graph_search(data, adj, q, 10, 32, n_threads=1)
+    b = oracle.ref_graph_search(data, adj, q, 10, 32, n_threads=4)
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
+
+
+def test_trace_invariants(oracle, song, datasets):
+    """Relations that hold on every reference query-stats CSV (visited = candidate +
+    seed, distance_evals = visited, slots = block * (expanded - early_stopped))."""
+    cfg = datasets.config1()
+    adj = song.nsw_build(cfg["base"], block=16, exact_order=False)
+    for ef in (10, 64, 200):
+        ids, d, tr = oracle.search(cfg["base"], adj, cfg["queries"], 10, ef, block=16,
+                                   bloom_bits=1 << 15, group=8)
+        assert (tr[:, 1] == tr[:, 4] + tr[:, 0]).all()
+        assert (tr[:, 5] == tr[:, 1]).all()
+        assert (tr[:, 3] == 16 * (tr[:, 2] - tr[:, 11])).all()
+        assert (tr[:, 0] == 4).all()
+        assert (tr[:, 8] <= ef).all() and (tr[:, 9] <= ef).all()
+        assert (tr[:, 10] == 10).all()
+        # best-first and consistent with the returned distances
+        assert (np.diff(d, axis=1) >= 0).all()
+        exact = ((cfg["queries"][:, None, :] - cfg["base"][ids]) ** 2).sum(-1)
+        np.testing.assert_array_equal(exact.astype(np.float32), d)
+
+
+# ---------------------------------------------------------------- Bloom filter
+
+MASK64 = (1 << 64) - 1
+BF_RAND = [0x4bcb391f924ed183, 0xa0ab69ccd854fc0a, 0x91086b9cecf5e3b7, 0xc68e01641bead407,
+           0x3a7b976128a30449, 0x6d122efabfc4d99f, 0xe6700ef8715030e2, 0x80dd0c3bffcfb45b]
+
+
+def py_bf_hash(h, x, nbits):
+    x ^= x >> 33
+    x = (x * BF_RAND[2 * h]) & MASK64
+    x ^= x >> 33
+    x = (x * BF_RAND[2 * h + 1]) & MASK64
+    x ^= x >> 33
+    return x % nbits
+
+
+@pytest.mark.parametrize("nbits", [512, 16384, 1 << 15, 1 << 16, 64 * 300])
+def test_bloom_hash_independent_restatement(oracle, nbits):
+    rng = np.random.default_rng(nbits)
+    ids = np.concatenate([np.arange(0, 200), rng.integers(0, 2**31 - 1, size=800)]).astype(np.int32)
+    got = oracle.bloom_positions(ids, nbits)
+    want = np.array([[py_bf_hash(h, int(i), nbits) for h in range(4)] for i in ids])
+    np.testing.assert_array_equal(got, want)
+    assert got.min() >= 0 and got.max() < nbits
+
+
+def test_bloom_layout_and_membership(oracle):
+    words = 256
+    ins = np.array([0, 1, 7, 123456, 999999], np.int32)
+    hit, bits = oracle.bloom_run(words, ins, ins)
+    assert hit.all()  # no false negatives
+    want = np.zeros(words, np.uint64)
+    for i in ins:
+        for h in range(4):
+            pos = py_bf_hash(h, int(i), words * 64)
+            want[pos % words] |= np.uint64(1) << np.uint64(pos // words)
+    np.testing.assert_array_equal(bits, want)
+
+
+def test_bloom_false_positive_rate(oracle):
+    """m = 1000 inserts into 2^14 bits: FPR close to (1 - e^{-4m/M})^4 (~1.3%)."""
+    words = 256
+    rng = np.random.default_rng(11)
+    ids = rng.permutation(2_000_000).astype(np.int32)
+    ins, probe = ids[:1000], ids[1000:201000]
+    hit, _ = oracle.bloom_run(words, ins, probe)
+    fpr = hit.mean()
+    theory = (1 - np.exp(-4 * 1000 / (words * 64))) ** 4
+    assert abs(fpr - theory) < 0.25 * theory, (fpr, theory)
+
+
+# ---------------------------------------------------------------- sets
+
+PUSH, POP_MIN, POP_WORST, WORST, DRAIN = 0, 1, 2, 3, 4
+
+
+def test_set_known_answers(oracle):
+    """Hand-derived from gpuHeap.h:23-36 (push), 39-49 (pop_min), 90-96 (pop_worst)."""
+    ops = [PUSH, PUSH, PUSH, PUSH, PUSH, PUSH, POP_MIN]
+    vals = [5.0, 9.0, 9.0, 1.0, 7.0, 9.0, 0.0]
+    ids = [50, 90, 91, 10, 70, 92, 0]
+    # cap 4: [5,9,9,1] full; push 7 replaces the FIRST 9 (index 1, id 90) -> [5,7,9,1];
+    # push 9 is not < max 9 -> rejected; pop_min takes index 3 (1.0, id 10), the last
+    # element fills it -> [5,7,9].
+    ret, val, fd, fi, _, _ = oracle.set_run(4, ops, vals, ids)
+    assert list(ret[:6]) == [1, 1, 1, 1, 1, 0]
+    assert ret[6] == 10 and val[6] == 1.0
+    np.testing.assert_array_equal(fd, [5.0, 7.0, 9.0])
+    np.testing.assert_array_equal(fi, [50, 70, 91])
+    # ties in pop_min: first index wins, last element moves into the hole
+    ret, val, fd, fi, _, _ = oracle.set_run(5, [PUSH] * 5 + [POP_MIN],
+                                            [3.0, 2.0, 4.0, 2.0, 8.0, 0.0], [1, 2, 3, 4, 5, 0])
+    assert ret[5] == 2
+    np.testing.assert_array_equal(fi, [1, 5, 3, 4])
+    # pop_worst on ties removes the first maximum; drain is best-first with
+    # first-index ties
+    ret, val, fd, fi, dr, dd = oracle.set_run(6, [PUSH] * 6 + [POP_WORST, DRAIN],
+                                              [4.0, 6.0, 1.0, 6.0, 1.0, 2.0, 0, 0],
+                                              [40, 60, 10, 61, 11, 20, 0, 4])
+    # after pop_worst: [4, 2(20 moved to index 1), 1, 6(61), 1(11)]
+    assert list(dr[:4]) == [10, 11, 20, 40]
+
+
+def test_set_invariants_random(oracle):
+    """Value-level model: pushes keep the cap smallest values (strict < to enter),
+    pop_min returns the minimum, worst the maximum.  Ids are checked to be a
+    subset of what was pushed; positions are covered by the known answers."""
+    rng = np.random.default_rng(3)
+    for trial in range(300):
+        cap = int(rng.integers(1, 40))
+        n = int(rng.integers(1, 120))
+        ops = rng.choice([PUSH, PUSH, PUSH, POP_MIN, WORST], size=n).astype(np.int32)
+        vals = rng.integers(0, 30, size=n).astype(np.float32)  # many ties
+        ids = np.arange(n, dtype=np.int32)
+        ret, val, fd, fi, _, _ = oracle.set_run(cap, ops, vals, ids)
+        model = []  # multiset of values currently held
+        for i in range(n):
+            if ops[i] == PUSH:
+                if len(model) < cap:
+                    model.append(vals[i])
+                    assert ret[i] == 1
+                else:
+                    mx = max(model)
+                    assert ret[i] == int(vals[i] < mx)
+                    if vals[i] < mx:
+                        model.remove(mx)
+                        model.append(vals[i])
+            elif ops[i] == POP_MIN:
+                if model:
+                    assert val[i] == min(model)
+                    assert vals[ret[i]] == val[i]
+                    model.remove(min(model))
+            elif ops[i] == WORST and model:
+                assert val[i] == max(model)
+        assert sorted(fd.tolist()) == sorted(float(v) for v in model)
+        assert len(set(fi.tolist())) == len(fi)
+        np.testing.assert_array_equal(vals[fi], fd)
+
+
+def test_drain_sorted_and_trim(oracle):
+    rng = np.random.default_rng(9)
+    for _ in range(100):
+        cap = int(rng.integers(1, 60))
+        n = int(rng.integers(1, 100))
+        vals = rng.random(n).astype(np.float32)
+        k = int(rng.integers(1, cap + 1))
+        ops = [PUSH] * n + [DRAIN]
+        ret, _, _, _, dr, dd = oracle.set_run(cap, ops, list(vals) + [0], list(range(n)) + [k])
+        cnt = ret[-1]
+        kept = np.sort(vals)[: min(cap, n)]  # distinct values: the cap smallest survive
+        np.testing.assert_array_equal(dd[:cnt], kept[:cnt])
+
+
+# ---------------------------------------------------------------- distances
+
+@REF_ONLY
+def test_sequential_sum_equals_reference_l2(oracle):
+    rng = np.random.default_rng(1)
+    for dim in (1, 7, 128, 960):
+        a = rng.standard_normal(dim).astype(np.float32) * 3
+        b = rng.standard_normal(dim).astype(np.float32)
+        assert np.float32(oracle.l2_sequential(a, b)).tobytes() == np.float32(oracle.ref_l2(a, b)).tobytes()
+
+
+@pytest.mark.parametrize("group", [1, 2, 4, 8, 16, 32, 64])
+def test_group_sum_orders(oracle, group):
+    rng = np.random.default_rng(group)
+    for dim in (1, 3, 4, 100, 128, 200, 960):
+        a = rng.integers(0, 256, size=dim).astype(np.float32)
+        b = rng.integers(0, 256, size=dim).astype(np.float32)
+        exact = int(((a.astype(np.int64) - b.astype(np.int64)) ** 2).sum())
+        if exact < 2 ** 24:
+            assert oracle.row_dist("l2", a, b, group) == exact
+        fa = rng.random(dim).astype(np.float32)
+        fb = rng.random(dim).astype(np.float32)
+        want = float(((fa.astype(np.float64) - fb) ** 2).sum())
+        assert abs(oracle.row_dist("l2", fa, fb, group) - want) <= 1e-5 * max(1.0, want)
+        wip = -float((fa.astype(np.float64) * fb).sum())
+        assert abs(oracle.row_dist("ip", fa, fb, group) - wip) <= 1e-5 * max(1.0, abs(wip))
+
+
+@REF_ONLY
+def test_hamming_matches_reference(oracle):
+    rng = np.random.default_rng(2)
+    codes = rng.integers(0, 2**63, size=(50, 4), dtype=np.uint64)
+    R = oracle.ref()
+    L = oracle.lib()
+    for i in range(49):
+        a, b = codes[i], codes[i + 1]
+        mine = L.or_hamming256(a.ctypes.data, b.ctypes.data)
+        theirs = R.ref_hamming(a.view(np.uint32).ctypes.data, b.view(np.uint32).ctypes.data)
+        want = int(np.unpackbits((a ^ b).view(np.uint8)).sum())
+        assert mine == theirs == want
+
+
+# ---------------------------------------------------------------- brute force
+
+def test_bruteforce_matches_numpy(oracle):
+    rng = np.random.default_rng(4)
+    base = rng.integers(0, 5, size=(800, 6)).astype(np.float32)
+    q = rng.integers(0, 5, size=(20, 6)).astype(np.float32)
+    ids, d = oracle.bruteforce(base, q, 7)
+    for i in range(q.shape[0]):
+        dist = ((base.astype(np.float64) - q[i]) ** 2).sum(1)
+        order = np.lexsort((np.arange(800), dist))[:7]
+        np.testing.assert_array_equal(ids[i], order)
+        np.testing.assert_array_equal(d[i], dist[order])
