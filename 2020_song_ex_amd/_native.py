@@ -36,7 +36,8 @@ class SongBatchTiming(ctypes.Structure):
 
 class SongSearchOpts(ctypes.Structure):
     _fields_ = [("bloom_bits", _I32), ("batch", _I32), ("n_seeds", _I32),
-                ("waves_per_block", _I32), ("lanes_per_row", _I32), ("reserved", _I32 * 11)]
+                ("waves_per_block", _I32), ("lanes_per_row", _I32), ("mode", _I32),
+                ("waves_per_cu", _I32), ("reserved", _I32 * 9)]
 
 
 EXPORTS = (
@@ -45,6 +46,7 @@ EXPORTS = (
     "song_bithash_project", "song_index_create", "song_index_destroy", "song_index_row_stride",
     "song_search", "song_search_device", "song_dev_alloc", "song_dev_free", "song_memcpy_h2d",
     "song_memcpy_d2h", "song_queries_to_device", "song_index_bruteforce",
+    "song_search_device_mixed",
 )
 
 _lib = None
@@ -85,6 +87,7 @@ def lib():
     L.song_memcpy_d2h.argtypes = [_P, _P, _I64]
     L.song_queries_to_device.argtypes = [_P, _P, _I32, ctypes.POINTER(_P)]
     L.song_index_bruteforce.argtypes = [_P, _P, _I32, _I32, _P, _P, _P]
+    L.song_search_device_mixed.argtypes = [_P, _P, _I32, _P, _P, _I32, _I32, _P, _P, _P, _P, _P]
     _lib = L
     return L
 
@@ -105,8 +108,14 @@ def device_count() -> int:
     return c.value
 
 
-def make_opts(bloom_bits=0, batch=0, n_seeds=0, waves_per_block=0, lanes_per_row=0):
+MODE_PLAIN, MODE_PERSISTENT = 0, 1
+
+
+def make_opts(bloom_bits=0, batch=0, n_seeds=0, waves_per_block=0, lanes_per_row=0, mode=0,
+              waves_per_cu=0):
     o = SongSearchOpts()
+    o.mode = mode
+    o.waves_per_cu = waves_per_cu
     o.bloom_bits = bloom_bits
     o.batch = batch
     o.n_seeds = n_seeds

@@ -67,6 +67,8 @@ struct song_index {
     float* d_dists = nullptr;
     int32_t* d_trace = nullptr;
     int64_t cap_q = 0, cap_ids = 0, cap_trace = 0;
+    int32_t* d_counter = nullptr;  // persistent-kernel queue head
+    int n_cus = 0;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     std::vector<char> stage;  // host staging for padded query rows
     std::mutex mu;
@@ -80,7 +82,8 @@ size_t host_row_bytes(const song_index* ix) {
 }
 
 struct Plan {
-    int ef, S, G, qch, wpb, bloom_words, bloom_shift, n_seeds, batch;
+    int ef, S, G, qch, bloom_words, bloom_shift, n_seeds, batch, persistent;
+    song::LaunchShape shape;
 };
 
 int make_plan(const song_index* ix, int32_t nq, int32_t k, int32_t ef, const song_search_opts* o, Plan* p) {
@@ -100,8 +103,14 @@ int make_plan(const song_index* ix, int32_t nq, int32_t k, int32_t ef, const son
     p->bloom_shift = ilog2(p->bloom_words);
     p->n_seeds = o->n_seeds ? o->n_seeds : 4;
     if (p->n_seeds < 1 || p->n_seeds > 64) return fail(SONG_ERR_ARG, "n_seeds must be in [1, 64]");
-    p->wpb = o->waves_per_block ? o->waves_per_block : 4;
-    if (p->wpb != 1 && p->wpb != 2 && p->wpb != 4) return fail(SONG_ERR_ARG, "waves_per_block must be 1, 2 or 4");
+    const int wpb = o->waves_per_block ? o->waves_per_block : 4;
+    if (wpb != 1 && wpb != 2 && wpb != 4) return fail(SONG_ERR_ARG, "waves_per_block must be 1, 2 or 4");
+    p->shape.waves_per_block = wpb;
+    if (o->mode != SONG_MODE_PLAIN && o->mode != SONG_MODE_PERSISTENT) return fail(SONG_ERR_ARG, "unknown mode");
+    p->persistent = o->mode == SONG_MODE_PERSISTENT;
+    const int wpc = o->waves_per_cu ? o->waves_per_cu : 32;
+    if (wpc < 1 || wpc > 32) return fail(SONG_ERR_ARG, "waves_per_cu must be in [1, 32]");
+    p->shape.persistent_blocks = std::max(1, (ix->n_cus * wpc + wpb - 1) / wpb);
     p->batch = o->batch > 0 ? o->batch : (nq > 0 ? nq : 1);
     if (o->batch < 0) return fail(SONG_ERR_ARG, "batch must be >= 0");
     if (ix->metric == SONG_METRIC_HAMMING) {
@@ -145,6 +154,10 @@ song::SearchArgs make_args(const song_index* ix, const Plan& p, const void* dq, 
     a.bloom_words = p.bloom_words;
     a.bloom_shift = p.bloom_shift;
     a.n_entry = p.n_seeds;
+    a.ks = nullptr;
+    a.efs = nullptr;
+    a.k_stride = k;
+    a.queue_head = p.persistent ? ix->d_counter : nullptr;
     return a;
 }
 
@@ -325,6 +338,10 @@ int song_index_create(const void* vecs, const int32_t* adj, int64_t n, int32_t d
     if (e != hipSuccess) return bail(hip_fail(e, "hipMalloc vectors"));
     e = hipMalloc(&ix->d_adj, (size_t)n * block * 4);
     if (e != hipSuccess) return bail(hip_fail(e, "hipMalloc adjacency"));
+    e = hipMalloc(&ix->d_counter, 64);
+    if (e != hipSuccess) return bail(hip_fail(e, "hipMalloc counter"));
+    e = hipDeviceGetAttribute(&ix->n_cus, hipDeviceAttributeMultiprocessorCount, device);
+    if (e != hipSuccess) return bail(hip_fail(e, "hipDeviceGetAttribute"));
     const size_t hb = host_row_bytes(ix);
     if (hb == rb) {
         e = hipMemcpy(ix->d_data, vecs, (size_t)n * rb, hipMemcpyHostToDevice);
@@ -356,6 +373,7 @@ void song_index_destroy(song_index* ix) {
     if (ix->d_ids) (void)hipFree(ix->d_ids);
     if (ix->d_dists) (void)hipFree(ix->d_dists);
     if (ix->d_trace) (void)hipFree(ix->d_trace);
+    if (ix->d_counter) (void)hipFree(ix->d_counter);
     for (auto& ev : ix->ev)
         if (ev) (void)hipEventDestroy(ev);
     if (ix->stream) (void)hipStreamDestroy(ix->stream);
@@ -395,7 +413,7 @@ int song_search(song_index* ix, const void* queries, int32_t nq, int32_t k, int3
         HIP_TRY(hipEventRecord(ix->ev[1], ix->stream), "hipEventRecord");
         song::SearchArgs a = make_args(ix, p, ix->d_q, bs, k, ix->d_ids, dists_out ? ix->d_dists : nullptr,
                                        trace_out ? ix->d_trace : nullptr);
-        HIP_TRY(song::launch_search(a, ix->metric, p.S, p.G, p.qch, p.wpb, ix->stream), "search kernel launch");
+        HIP_TRY(song::launch_search(a, ix->metric, p.S, p.G, p.qch, p.shape, ix->stream), "search kernel launch");
         HIP_TRY(hipEventRecord(ix->ev[2], ix->stream), "hipEventRecord");
         HIP_TRY(hipMemcpyAsync(ids_out + off * k, ix->d_ids, (size_t)bs * k * 4, hipMemcpyDeviceToHost, ix->stream),
                 "ids D2H");
@@ -455,7 +473,7 @@ int song_search_device(song_index* ix, const void* d_queries, int32_t nq, int32_
         song::SearchArgs a = make_args(ix, p, (const char*)d_queries + off * rb, bs, k, d_ids + off * k,
                                        d_dists ? d_dists + off * k : nullptr,
                                        d_trace ? (int32_t*)(d_trace + off) : nullptr);
-        HIP_TRY(song::launch_search(a, ix->metric, p.S, p.G, p.qch, p.wpb, ix->stream), "search kernel launch");
+        HIP_TRY(song::launch_search(a, ix->metric, p.S, p.G, p.qch, p.shape, ix->stream), "search kernel launch");
     }
     HIP_TRY(hipEventRecord(ix->ev[2], ix->stream), "hipEventRecord");
     HIP_TRY(hipEventSynchronize(ix->ev[2]), "search synchronize");
@@ -464,6 +482,38 @@ int song_search_device(song_index* ix, const void* d_queries, int32_t nq, int32_
     tm.kernel_ms = ker;
     tm.total_ms = ms_since(t0);
     tm.n_batches = bi;
+    if (timing_out) *timing_out = tm;
+    return SONG_OK;
+}
+
+int song_search_device_mixed(song_index* ix, const void* d_queries, int32_t nq, const int32_t* d_ks,
+                             const int32_t* d_efs, int32_t k_max, int32_t ef_max, const song_search_opts* opts,
+                             int32_t* d_ids, float* d_dists, song_query_trace* d_trace, song_timing* timing_out) {
+    if (!ix) return fail(SONG_ERR_ARG, "null index");
+    if (nq > 0 && (!d_queries || !d_ids || !d_ks || !d_efs)) return fail(SONG_ERR_ARG, "null buffer");
+    Plan p;
+    int rc = make_plan(ix, nq, k_max, ef_max, opts, &p);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lock(ix->mu);
+    HIP_TRY(hipSetDevice(ix->device), "hipSetDevice");
+    song_timing tm;
+    std::memset(&tm, 0, sizeof tm);
+    tm.ef_used = p.ef;
+    auto t0 = std::chrono::steady_clock::now();
+    HIP_TRY(hipEventRecord(ix->ev[1], ix->stream), "hipEventRecord");
+    if (nq > 0) {
+        song::SearchArgs a = make_args(ix, p, d_queries, nq, k_max, d_ids, d_dists, (int32_t*)d_trace);
+        a.ks = d_ks;
+        a.efs = d_efs;
+        HIP_TRY(song::launch_search(a, ix->metric, p.S, p.G, p.qch, p.shape, ix->stream), "search kernel launch");
+    }
+    HIP_TRY(hipEventRecord(ix->ev[2], ix->stream), "hipEventRecord");
+    HIP_TRY(hipEventSynchronize(ix->ev[2]), "search synchronize");
+    float ker = 0;
+    HIP_TRY(hipEventElapsedTime(&ker, ix->ev[1], ix->ev[2]), "hipEventElapsedTime");
+    tm.kernel_ms = ker;
+    tm.total_ms = ms_since(t0);
+    tm.n_batches = nq > 0 ? 1 : 0;
     if (timing_out) *timing_out = tm;
     return SONG_OK;
 }

@@ -24,11 +24,20 @@ struct SearchArgs {
     int32_t bloom_words;     // 64-bit words, power of two
     int32_t bloom_shift;     // log2(bloom_words)
     int32_t n_entry;         // seeds per query (<= 64)
+    const int32_t* ks;       // per-query k, or null (uniform k)
+    const int32_t* efs;      // per-query queue size, or null (uniform ef); ef := max(ef, k)
+    int32_t k_stride;        // output row length (>= every k)
+    int32_t* queue_head;     // persistent mode: device counter (zeroed at launch), else null
+};
+
+struct LaunchShape {
+    int waves_per_block;     // 1, 2 or 4 queries per workgroup
+    int persistent_blocks;   // grid size in persistent mode
 };
 
 // metric: METRIC_*; S: set slots per lane (cap <= 64*S, S in {1,2,4,8});
 // G: lanes per row (power of two); qch: 16-byte chunks per lane (G*qch*4 >= dstride).
-hipError_t launch_search(const SearchArgs& a, int metric, int S, int G, int qch, int waves_per_block,
+hipError_t launch_search(const SearchArgs& a, int metric, int S, int G, int qch, const LaunchShape& shape,
                          hipStream_t st);
 
 }  // namespace song

@@ -379,23 +379,22 @@ __device__ __forceinline__ void hamming_distances(const ulonglong2* __restrict__
     }
 }
 
-// ---------------------------------------------------------------- the kernel
+// ---------------------------------------------------------------- one query
 
+// The search of query qid by the calling wave (kg_search_kernel body,
+// kernalGraph.h:192-298).  lds: this wave's bloom[W] | stage_id[64] | stage_d[64].
 template <int METRIC, int S, int G, int QCH>
-__global__ void __launch_bounds__(256) search_kernel(SearchArgs P) {
-    extern __shared__ unsigned long long smem[];
-    const int lane = lane_id();
-    const int wave = threadIdx.x >> 6;
-    const int qid = blockIdx.x * (blockDim.x >> 6) + wave;
-    if (qid >= P.nq) return;
-
-    // LDS per wave: bloom[W] | stage_id[64] | stage_d[64]
-    unsigned long long* bloom = smem + (size_t)wave * (P.bloom_words + 64);
+__device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int lane, unsigned long long* lds) {
+    unsigned long long* bloom = lds;
     int* stage_id = (int*)(bloom + P.bloom_words);
     float* stage_d = (float*)(stage_id + 64);
     const uint32_t wmask = (uint32_t)P.bloom_words - 1u;
     const int wshift = P.bloom_shift;
     const uint32_t nbits_mask = (uint32_t)P.bloom_words * 64u - 1u;
+    // per-query budgets (serving / persistent queues mix them); ef >= k
+    // (clamped so a bad device-side budget cannot overrun the register sets)
+    const int k = min(P.ks ? P.ks[qid] : P.k, P.k_stride);
+    const int ef = min(P.efs ? max(P.efs[qid], k) : P.ef, 64 * S);
 
     for (int w = lane; w < P.bloom_words; w += 64) bloom[w] = 0ull;
 
@@ -417,23 +416,15 @@ __global__ void __launch_bounds__(256) search_kernel(SearchArgs P) {
     wave_lds_sync();
 
     WSet<S> cand, res;
-    cand.init(P.ef);
-    res.init(P.ef);
+    cand.init(ef);
+    res.init(ef);
     // QueryTrace, telemetry.h:5-18 (wave-uniform counters)
     int tr_seed = 0, tr_visited = 0, tr_expanded = 0, tr_slots = 0, tr_cand = 0, tr_devals = 0;
     int tr_fpush = 0, tr_rupd = 0, tr_peakf = 0, tr_peakr = 0, tr_early = 0;
 
-    auto distances = [&](int F) {
-        if constexpr (METRIC == METRIC_HAMMING) {
-            hamming_distances((const ulonglong2*)P.data, qc, F, stage_id, stage_d, lane);
-        } else {
-            float_distances<METRIC, G, QCH, 2>((const float4*)P.data, d4, qf, F, stage_id, stage_d, lane);
-        }
-    };
-
     // Bloom-test a row of ids (lane order), compute the distances of the new
     // ones and push them to both sets in order.  Returns the number of new ids.
-    auto visit_row = [&](int id, bool valid) -> int {
+    auto visit_row = [&](int id, bool valid) __attribute__((always_inline)) -> int {
         const uint64_t acc = bloom_row(bloom, wmask, wshift, nbits_mask, id, valid);
         const int F = __popcll(acc);
         if (F == 0) return 0;
@@ -441,7 +432,11 @@ __global__ void __launch_bounds__(256) search_kernel(SearchArgs P) {
                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)acc, 0u));
         if ((acc >> lane) & 1ull) stage_id[rank] = id;
         wave_lds_sync();
-        distances(F);
+        if constexpr (METRIC == METRIC_HAMMING) {
+            hamming_distances((const ulonglong2*)P.data, qc, F, stage_id, stage_d, lane);
+        } else {
+            float_distances<METRIC, G, QCH, 2>((const float4*)P.data, d4, qf, F, stage_id, stage_d, lane);
+        }
         wave_lds_sync();
         const float it_d = lane < F ? stage_d[lane] : INFINITY;
         const int it_id = lane < F ? stage_id[lane] : -1;
@@ -498,13 +493,13 @@ __global__ void __launch_bounds__(256) search_kernel(SearchArgs P) {
     }
 
     // Trim to k, then drain best-first (kernalGraph.h:292-298, gpuHeap.h:90-108).
-    while (res.sz > P.k) {
+    while (res.sz > k) {
         const float M = res.vmax(lane);
         res.remove_at(res.first_eq(M, lane), lane);
     }
-    const int cnt = res.sz < P.k ? res.sz : P.k;
-    int32_t* out = P.ids_out + (size_t)qid * P.k;
-    float* dout = P.dists_out ? P.dists_out + (size_t)qid * P.k : nullptr;
+    const int cnt = res.sz < k ? res.sz : k;
+    int32_t* out = P.ids_out + (size_t)qid * P.k_stride;
+    float* dout = P.dists_out ? P.dists_out + (size_t)qid * P.k_stride : nullptr;
     for (int i = 0; i < cnt; i++) {
         const float m = res.vmin(lane);
         const int p = res.first_eq(m, lane);
@@ -517,7 +512,7 @@ __global__ void __launch_bounds__(256) search_kernel(SearchArgs P) {
             if (dout) dout[i] = dv;
         }
     }
-    for (int i = cnt + lane; i < P.k; i += 64) {
+    for (int i = cnt + lane; i < P.k_stride; i += 64) {
         out[i] = -1;
         if (dout) dout[i] = INFINITY;
     }
@@ -536,21 +531,61 @@ __global__ void __launch_bounds__(256) search_kernel(SearchArgs P) {
         t[10] = cnt;
         t[11] = tr_early;
     }
+    wave_lds_sync();
+}
+
+// ---------------------------------------------------------------- kernels
+
+// One wave per query (the reference's one block per query, kernalGraph.h:354).
+template <int METRIC, int S, int G, int QCH>
+__global__ void __launch_bounds__(256) search_kernel(SearchArgs P) {
+    extern __shared__ unsigned long long smem[];
+    const int wave = threadIdx.x >> 6;
+    const int qid = blockIdx.x * (blockDim.x >> 6) + wave;
+    if (qid >= P.nq) return;
+    search_query<METRIC, S, G, QCH>(P, qid, lane_id(), smem + (size_t)wave * (P.bloom_words + 64));
+}
+
+// Persistent: a fixed grid of waves drains the query queue; each wave takes
+// the next query index from a device counter until all nq are taken.  Every
+// wave leaves once the counter passes nq, so the grid always drains.
+template <int METRIC, int S, int G, int QCH>
+__global__ void __launch_bounds__(256) persistent_kernel(SearchArgs P) {
+    extern __shared__ unsigned long long smem[];
+    const int wave = threadIdx.x >> 6;
+    const int lane = lane_id();
+    unsigned long long* lds = smem + (size_t)wave * (P.bloom_words + 64);
+    for (;;) {
+        int item = 0;
+        if (lane == 0) item = atomicAdd(P.queue_head, 1);
+        item = __builtin_amdgcn_readfirstlane(item);
+        if (item >= P.nq) break;
+        search_query<METRIC, S, G, QCH>(P, item, lane, lds);
+    }
+}
+
+__global__ void zero_counter_kernel(int* c) {
+    if (threadIdx.x == 0) *c = 0;
 }
 
 // ---------------------------------------------------------------- dispatch
 
 template <int METRIC, int S, int G, int QCH>
-static hipError_t launch_one(const SearchArgs& a, int waves_per_block, hipStream_t st) {
-    const int threads = 64 * waves_per_block;
-    const int blocks = (a.nq + waves_per_block - 1) / waves_per_block;
-    const size_t lds = (size_t)waves_per_block * ((size_t)a.bloom_words + 64) * 8;
-    hipLaunchKernelGGL((search_kernel<METRIC, S, G, QCH>), dim3(blocks), dim3(threads), lds, st, a);
+static hipError_t launch_one(const SearchArgs& a, const LaunchShape& ls, hipStream_t st) {
+    const int threads = 64 * ls.waves_per_block;
+    const size_t lds = (size_t)ls.waves_per_block * ((size_t)a.bloom_words + 64) * 8;
+    if (a.queue_head) {
+        hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(64), 0, st, a.queue_head);
+        hipLaunchKernelGGL((persistent_kernel<METRIC, S, G, QCH>), dim3(ls.persistent_blocks), dim3(threads), lds, st, a);
+    } else {
+        const int blocks = (a.nq + ls.waves_per_block - 1) / ls.waves_per_block;
+        hipLaunchKernelGGL((search_kernel<METRIC, S, G, QCH>), dim3(blocks), dim3(threads), lds, st, a);
+    }
     return hipGetLastError();
 }
 
 template <int METRIC, int S>
-static hipError_t launch_g(const SearchArgs& a, int G, int qch, int wpb, hipStream_t st) {
+static hipError_t launch_g(const SearchArgs& a, int G, int qch, const LaunchShape& wpb, hipStream_t st) {
     if constexpr (METRIC == METRIC_HAMMING) {
         return launch_one<METRIC, S, 2, 1>(a, wpb, st);
     } else {
@@ -573,7 +608,7 @@ static hipError_t launch_g(const SearchArgs& a, int G, int qch, int wpb, hipStre
 }
 
 template <int METRIC>
-static hipError_t launch_s(const SearchArgs& a, int S, int G, int qch, int wpb, hipStream_t st) {
+static hipError_t launch_s(const SearchArgs& a, int S, int G, int qch, const LaunchShape& wpb, hipStream_t st) {
     switch (S) {
     case 1: return launch_g<METRIC, 1>(a, G, qch, wpb, st);
     case 2: return launch_g<METRIC, 2>(a, G, qch, wpb, st);
@@ -583,7 +618,7 @@ static hipError_t launch_s(const SearchArgs& a, int S, int G, int qch, int wpb, 
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_search(const SearchArgs& a, int metric, int S, int G, int qch, int wpb, hipStream_t st) {
+hipError_t launch_search(const SearchArgs& a, int metric, int S, int G, int qch, const LaunchShape& wpb, hipStream_t st) {
     switch (metric) {
     case METRIC_L2: return launch_s<METRIC_L2>(a, S, G, qch, wpb, st);
     case METRIC_IP: return launch_s<METRIC_IP>(a, S, G, qch, wpb, st);

@@ -33,36 +33,6 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def gpu_ground_truth(base, queries, k):
-    """Exact k-NN by brute force on the GPU (fp32 GEMM; exact for integer data, dim <= 258)."""
-    import torch
-
-    dev = torch.device("cuda:0")
-    b = torch.from_numpy(base).to(dev)
-    q = torch.from_numpy(queries).to(dev)
-    bn = (b * b).sum(1)
-    qn = (q * q).sum(1)
-    best_d = None
-    best_i = None
-    step = 1 << 17
-    for s in range(0, b.shape[0], step):
-        bb = b[s:s + step]
-        d = qn[:, None] + bn[None, s:s + step] - 2.0 * (q @ bb.T)
-        dv, di = torch.topk(d, k, dim=1, largest=False)
-        di = di + s
-        if best_d is None:
-            best_d, best_i = dv, di
-        else:
-            cd = torch.cat([best_d, dv], 1)
-            ci = torch.cat([best_i, di], 1)
-            best_d, sel = torch.topk(cd, k, dim=1, largest=False)
-            best_i = torch.gather(ci, 1, sel)
-    out = best_i.int().cpu().numpy()
-    del b, q
-    torch.cuda.empty_cache()
-    return out
-
-
 def recall_at(ids, gt, k):
     hits = 0
     for a, g in zip(ids[:, :k], gt[:, :k]):
@@ -131,8 +101,8 @@ def main():
     d_tr = N.DeviceBuffer(nq * 48)
 
     t0 = time.time()
-    gt = gpu_ground_truth(base, queries, kmax)
-    log(f"brute-force ground truth in {time.time() - t0:.1f}s")
+    gt, _, gt_ms = idx.bruteforce(queries, kmax)
+    log(f"brute-force ground truth on the GPU in {time.time() - t0:.1f}s (kernels {gt_ms:.0f} ms)")
 
     def run(k, ef, trace=True):
         tm = idx.search_device(dq, nq, k, ef, d_ids, None, d_tr if trace else None)

@@ -106,9 +106,17 @@ typedef struct song_search_opts {
     int32_t waves_per_block; /* 1, 2 or 4 queries per workgroup; 0 = 4 */
     int32_t lanes_per_row;   /* lanes that share one vector row (power of
                                 two, 2..64); 0 = chosen from dim.  Fixes the
-                                float summation order (see DESIGN). */
-    int32_t reserved[11];
+                                float summation order (xor butterfly over
+                                the group, see song_kernels.hip). */
+    int32_t mode;            /* SONG_MODE_PLAIN: one wave per query;
+                                SONG_MODE_PERSISTENT: a fixed grid drains a
+                                device query queue.  Same results. */
+    int32_t waves_per_cu;    /* persistent grid: waves per compute unit,
+                                1..32; 0 = 32 */
+    int32_t reserved[9];
 } song_search_opts;
+
+enum { SONG_MODE_PLAIN = 0, SONG_MODE_PERSISTENT = 1 };
 
 /* Version of this header's ABI (SONG_ABI_VERSION). */
 int song_abi_version(void);
@@ -205,6 +213,16 @@ int song_search_device(song_index* index, const void* d_queries, int32_t nq, int
  * kernel_ms_out (nullable): HIP-event time of the two kernels. */
 int song_index_bruteforce(song_index* index, const void* queries, int32_t nq, int32_t k,
                           int32_t* ids_out, float* dists_out, double* kernel_ms_out);
+
+/* Device-resident search with a budget per query (mixed k and queue sizes in
+ * one launch, as a serving queue holds them).  d_ks / d_efs: int32 [nq] on the
+ * device; each ef is raised to its k; k_max >= every k, ef_max >= every ef
+ * (<= 512).  Output rows are k_max long, -1 padded.  No counterpart in the
+ * reference (one k and ef per gpu_search call, kernalGraph.h:307-309). */
+int song_search_device_mixed(song_index* index, const void* d_queries, int32_t nq, const int32_t* d_ks,
+                             const int32_t* d_efs, int32_t k_max, int32_t ef_max,
+                             const song_search_opts* opts, int32_t* d_ids, float* d_dists,
+                             song_query_trace* d_trace, song_timing* timing_out);
 
 /* ---------------------------------------------------------------- device memory helpers */
 

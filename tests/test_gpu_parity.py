@@ -75,10 +75,12 @@ def test_bloom_sizes(song, oracle, c1, bits):
     assert_same(r, o, f"bloom={bits}")
 
 
-@pytest.mark.parametrize("lanes", [2, 4, 8, 16, 32, 64])
-def test_lane_groups(song, oracle, datasets, lanes):
-    """Float data: every lane-group size gives the oracle's distances bit for bit."""
-    base, q = datasets.beta_mix(3000, 40, dim=100, n_centres=20, seed=3)
+@pytest.mark.parametrize("lanes,dim", [(2, 30), (4, 30), (8, 30), (16, 30), (32, 30), (64, 30),
+                                       (8, 100), (16, 100), (64, 100)])
+def test_lane_groups(song, oracle, datasets, lanes, dim):
+    """Float data: every lane-group size gives the oracle's distances bit for bit
+    (a group of G lanes holds 4 chunks each, so G * 16 >= dim or G = 64)."""
+    base, q = datasets.beta_mix(3000, 40, dim=dim, n_centres=20, seed=3)
     adj = song.nsw_build(base, block=32, exact_order=False)
     r, o = run_both(song, oracle, base, adj, q, 10, 48, lanes=lanes)
     assert_same(r, o, f"lanes={lanes}")
@@ -150,6 +152,62 @@ def test_duplicate_ids_in_row(song, oracle, c1):
     adj2[:, 8:] = adj2[:, :8]
     r, o = run_both(song, oracle, cfg["base"], adj2, cfg["queries"][:50], 10, 64)
     assert_same(r, o, "dup rows")
+
+
+def test_lane_group_too_small_is_an_error(song, c1):
+    cfg, adj = c1
+    idx = song.Index(cfg["base"], adj)
+    with pytest.raises(song.SongError, match="lanes_per_row"):
+        idx.search(cfg["queries"][:4], 10, 32, lanes_per_row=2)
+    idx.close()
+
+
+def test_persistent_equals_plain(song, oracle, c1):
+    """The persistent work-queue kernel returns the plain kernel's results for every grid size."""
+    cfg, adj = c1
+    idx = song.Index(cfg["base"], adj)
+    plain = idx.search(cfg["queries"], 10, 64)
+    for wpc in (1, 3, 8, 32):
+        for wpb in (1, 4):
+            r = idx.search(cfg["queries"], 10, 64, mode=1, waves_per_cu=wpc, waves_per_block=wpb)
+            np.testing.assert_array_equal(r.ids, plain.ids)
+            np.testing.assert_array_equal(r.dists, plain.dists)
+            np.testing.assert_array_equal(r.trace, plain.trace)
+    idx.close()
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_mixed_budgets_match_uniform_runs(song, c1, mode):
+    """Per-query k and queue size in one launch == separate uniform searches."""
+    cfg, adj = c1
+    q = cfg["queries"]
+    nq = q.shape[0]
+    rng = np.random.default_rng(0)
+    ks = rng.choice([1, 10, 100], size=nq).astype(np.int32)
+    efs = rng.choice([16, 64, 256], size=nq).astype(np.int32)
+    idx = song.Index(cfg["base"], adj)
+    N = song._native
+    dq = idx.queries_to_device(q)
+    d_ks = N.DeviceBuffer(nq * 4).upload(ks)
+    d_efs = N.DeviceBuffer(nq * 4).upload(efs)
+    d_ids = N.DeviceBuffer(nq * 100 * 4)
+    d_d = N.DeviceBuffer(nq * 100 * 4)
+    d_tr = N.DeviceBuffer(nq * 48)
+    idx.search_device_mixed(dq, nq, d_ks, d_efs, 100, 256, d_ids, d_d, d_tr, mode=mode)
+    ids = d_ids.download(np.empty((nq, 100), np.int32))
+    dd = d_d.download(np.empty((nq, 100), np.float32))
+    tr = d_tr.download(np.empty((nq, 12), np.int32))
+    for kk in (1, 10, 100):
+        for ee in (16, 64, 256):
+            sel = np.where((ks == kk) & (efs == ee))[0]
+            if sel.size == 0:
+                continue
+            r = idx.search(q[sel], kk, ee)
+            np.testing.assert_array_equal(ids[sel, :kk], r.ids)
+            np.testing.assert_array_equal(dd[sel, :kk], r.dists)
+            assert (ids[sel, kk:] == -1).all()
+            np.testing.assert_array_equal(tr[sel], r.trace)
+    idx.close()
 
 
 def test_empty_query_set(song, c1):
