@@ -46,7 +46,7 @@ EXPORTS = (
     "song_bithash_project", "song_index_create", "song_index_destroy", "song_index_row_stride",
     "song_search", "song_search_device", "song_dev_alloc", "song_dev_free", "song_memcpy_h2d",
     "song_memcpy_d2h", "song_queries_to_device", "song_index_bruteforce",
-    "song_search_device_mixed",
+    "song_search_device_mixed", "song_dev_copy",
 )
 
 _lib = None
@@ -87,6 +87,7 @@ def lib():
     L.song_memcpy_d2h.argtypes = [_P, _P, _I64]
     L.song_queries_to_device.argtypes = [_P, _P, _I32, ctypes.POINTER(_P)]
     L.song_index_bruteforce.argtypes = [_P, _P, _I32, _I32, _P, _P, _P]
+    L.song_dev_copy.argtypes = [_P, _P, _I64, _P]
     L.song_search_device_mixed.argtypes = [_P, _P, _I32, _P, _P, _I32, _I32, _P, _P, _P, _P, _P]
     _lib = L
     return L

@@ -76,6 +76,13 @@ struct song_index {
 
 namespace {
 
+// Plain streaming copy, 16 bytes per lane per step (the reference traffic for
+// calibrating hardware byte counters against a known number of bytes).
+__global__ void copy16_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n16) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 size_t row_bytes(const song_index* ix) { return (size_t)ix->dstride * (ix->metric == SONG_METRIC_HAMMING ? 8 : 4); }
 size_t host_row_bytes(const song_index* ix) {
     return ix->metric == SONG_METRIC_HAMMING ? 32 : (size_t)ix->dim * 4;
@@ -137,7 +144,7 @@ int make_plan(const song_index* ix, int32_t nq, int32_t k, int32_t ef, const son
 
 song::SearchArgs make_args(const song_index* ix, const Plan& p, const void* dq, int32_t nq, int32_t k,
                            int32_t* ids, float* dists, int32_t* trace) {
-    song::SearchArgs a;
+    song::SearchArgs a = {};
     a.data = ix->d_data;
     a.adj = ix->d_adj;
     a.queries = dq;
@@ -553,7 +560,7 @@ int song_index_bruteforce(song_index* ix, const void* queries, int32_t nq, int32
     }
     const void* src = stage_rows(ix, queries, nq);
     e = hipMemcpyAsync(dq, src, (size_t)nq * rb, hipMemcpyHostToDevice, ix->stream);
-    song::GtArgs a;
+    song::GtArgs a = {};
     a.data = ix->d_data;
     a.queries = dq;
     a.partial = partial;
@@ -590,6 +597,26 @@ int song_dev_alloc(int32_t device, int64_t bytes, void** out) {
 
 int song_dev_free(void* p) {
     if (p) HIP_TRY(hipFree(p), "hipFree");
+    return SONG_OK;
+}
+
+int song_dev_copy(void* dst, const void* src, int64_t bytes, double* kernel_ms_out) {
+    if (!dst || !src || bytes < 0 || (bytes % 16) != 0) return fail(SONG_ERR_ARG, "bytes must be a multiple of 16");
+    hipEvent_t a, b;
+    HIP_TRY(hipEventCreate(&a), "hipEventCreate");
+    HIP_TRY(hipEventCreate(&b), "hipEventCreate");
+    const int64_t n16 = bytes / 16;
+    const int blocks = (int)std::min<int64_t>(8192, std::max<int64_t>(1, (n16 + 255) / 256));
+    HIP_TRY(hipEventRecord(a, 0), "hipEventRecord");
+    hipLaunchKernelGGL(copy16_kernel, dim3(blocks), dim3(256), 0, 0, (const uint4*)src, (uint4*)dst, n16);
+    HIP_TRY(hipGetLastError(), "copy kernel launch");
+    HIP_TRY(hipEventRecord(b, 0), "hipEventRecord");
+    HIP_TRY(hipEventSynchronize(b), "hipEventSynchronize");
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (kernel_ms_out) *kernel_ms_out = ms;
     return SONG_OK;
 }
 

@@ -8,6 +8,41 @@
 
 namespace song {
 
+// Learned entry and termination (2 x 64 hidden FP32 MLPs).
+constexpr int LM_P = 16;  // query projection dims (entry MLP input)
+constexpr int LM_C = 64;  // candidate entry vertices (entry MLP outputs)
+constexpr int LM_F = 8;   // per-hop features (stop MLP input)
+constexpr int LM_H = 64;  // hidden width (one lane per unit)
+constexpr int LM_STOP_FLOATS = LM_H * LM_F + LM_H + LM_H * LM_H + LM_H + LM_H;  // w1 b1 w2 b2 w3
+
+struct LearnedArgs {
+    // entry: z = scale * (proj (q - mu)); scores = W3 relu(W2 relu(W1 z + b1) + b2) + b3
+    const float* mu;          // [dstride] (float metrics; unused for Hamming)
+    const float* proj;        // [LM_P x proj_dim] row-major, proj_dim = dstride or 256 (Hamming, +-1 bits)
+    const float* proj_scale;  // [LM_P]
+    const float* e_w1;        // [LM_H x LM_P]
+    const float* e_b1;        // [LM_H]
+    const float* e_w2;        // [LM_H x LM_H]
+    const float* e_b2;        // [LM_H]
+    const float* e_w3;        // [LM_C x LM_H]
+    const float* e_b3;        // [LM_C]
+    const int32_t* pivots;    // [LM_C] node ids (< n, checked on the host)
+    // stop: logit = w3 . relu(W2 relu(W1 f + b1) + b2) + b3, features f (LM_F)
+    const float* s_weights;   // LM_STOP_FLOATS: w1 [H x F], b1 [H], w2 [H x H], b2 [H], w3 [H]
+    float s_b3;
+    float stop_logit;         // stop when logit > stop_logit
+    float dist_scale;         // feature normalisation of distance differences
+    int32_t use_entry;
+    int32_t use_stop;
+    int32_t min_hops;         // first hop at which the stop MLP is consulted
+    int32_t interval;         // consult every `interval` hops
+    // training traces (all null outside trace mode)
+    float* trace;             // [nq x max_evals x (LM_F + 1)]: features, k-th best distance
+    int32_t* trace_count;     // [nq]
+    float* final_dk;          // [nq] k-th best distance at the end of the search
+    int32_t max_evals;
+};
+
 // Everything one launch of the search kernel reads or writes.
 struct SearchArgs {
     const void* data;        // f32 [n x dstride] (L2/IP) or u64 [n x 4] (Hamming)
@@ -28,6 +63,8 @@ struct SearchArgs {
     const int32_t* efs;      // per-query queue size, or null (uniform ef); ef := max(ef, k)
     int32_t k_stride;        // output row length (>= every k)
     int32_t* queue_head;     // persistent mode: device counter (zeroed at launch), else null
+    int32_t learned;         // 1: use L (entry / stop / trace as its flags say)
+    LearnedArgs L;
 };
 
 struct LaunchShape {

@@ -76,6 +76,16 @@ __device__ __forceinline__ float wave_min(float v) {
     return fminf(a, b);
 }
 
+__device__ __forceinline__ float wave_sum(float v) {
+    v = v + dppf<DPP_XOR1>(v);
+    v = v + dppf<DPP_XOR2>(v);
+    v = v + dppf<DPP_HALF_MIRROR>(v);
+    v = v + dppf<DPP_MIRROR>(v);
+    const float a = readlane_f(v, 0) + readlane_f(v, 16);
+    const float b = readlane_f(v, 32) + readlane_f(v, 48);
+    return a + b;
+}
+
 // Sum over aligned groups of G lanes, xor-butterfly order: for s = 1, 2, ...
 // < G every lane t takes acc[t] + acc[t ^ s].  Steps 4 and 8 use the mirror
 // patterns, whose partner lies in the same already-uniform 4 / 8 block as the
@@ -379,12 +389,162 @@ __device__ __forceinline__ void hamming_distances(const ulonglong2* __restrict__
     }
 }
 
+// ---------------------------------------------------------------- learned entry / termination
+
+// One dense 64 -> 64 layer, one output unit per lane: lane j returns
+// b[j] + sum_i W[j][i] x_i, where x_i is held by lane i.  W row-major in global
+// memory (read-only, shared by every wave, so it stays in cache).
+__device__ __forceinline__ float dense64_global(const float* __restrict__ W, const float* __restrict__ b, float x,
+                                                int lane) {
+    const float4* row = (const float4*)(W + lane * 64);
+    float acc = b[lane];
+#pragma unroll
+    for (int i4 = 0; i4 < 16; i4++) {
+        const float4 w = row[i4];
+        acc = __builtin_fmaf(w.x, readlane_f(x, 4 * i4 + 0), acc);
+        acc = __builtin_fmaf(w.y, readlane_f(x, 4 * i4 + 1), acc);
+        acc = __builtin_fmaf(w.z, readlane_f(x, 4 * i4 + 2), acc);
+        acc = __builtin_fmaf(w.w, readlane_f(x, 4 * i4 + 3), acc);
+    }
+    return acc;
+}
+
+// The stop network's weights, staged once per workgroup in LDS: w1 [64][8],
+// b1 [64], w2 interleaved so lane j reads 16 conflict-free float4
+// (element (j, i) at (i/4)*256 + j*4 + i%4), b2 [64], w3 [64].
+// Offsets of the stop network's arrays in s_weights (and in LDS).
+constexpr int SW_B1 = LM_H * LM_F;         // b1 [64]
+constexpr int SW_W2 = SW_B1 + LM_H;        // w2 [64][64]
+constexpr int SW_B2 = SW_W2 + LM_H * LM_H; // b2 [64]
+constexpr int SW_W3 = SW_B2 + LM_H;        // w3 [64]
+
+__device__ __forceinline__ void stage_stop_weights(const float* __restrict__ src, float* lds) {
+    for (int i = threadIdx.x; i < LM_STOP_FLOATS; i += blockDim.x) {
+        int dst = i;
+        if (i >= SW_W2 && i < SW_B2) {
+            const int e = i - SW_W2, j = e / LM_H, c = e % LM_H;
+            dst = SW_W2 + (c >> 2) * 256 + j * 4 + (c & 3);
+        }
+        lds[dst] = src[i];
+    }
+}
+
+// logit of the stop network for wave-uniform features f[0..7].
+__device__ __forceinline__ float stop_logit(const float* lds, const float (&f)[LM_F], float b3, int lane) {
+    const float4 wa = *(const float4*)(lds + lane * LM_F);
+    const float4 wb = *(const float4*)(lds + lane * LM_F + 4);
+    float h = lds[SW_B1 + lane];
+    h = __builtin_fmaf(wa.x, f[0], h);
+    h = __builtin_fmaf(wa.y, f[1], h);
+    h = __builtin_fmaf(wa.z, f[2], h);
+    h = __builtin_fmaf(wa.w, f[3], h);
+    h = __builtin_fmaf(wb.x, f[4], h);
+    h = __builtin_fmaf(wb.y, f[5], h);
+    h = __builtin_fmaf(wb.z, f[6], h);
+    h = __builtin_fmaf(wb.w, f[7], h);
+    h = fmaxf(h, 0.f);
+    float acc = lds[SW_B2 + lane];
+#pragma unroll
+    for (int i4 = 0; i4 < 16; i4++) {
+        const float4 w = *(const float4*)(lds + SW_W2 + i4 * 256 + lane * 4);
+        acc = __builtin_fmaf(w.x, readlane_f(h, 4 * i4 + 0), acc);
+        acc = __builtin_fmaf(w.y, readlane_f(h, 4 * i4 + 1), acc);
+        acc = __builtin_fmaf(w.z, readlane_f(h, 4 * i4 + 2), acc);
+        acc = __builtin_fmaf(w.w, readlane_f(h, 4 * i4 + 3), acc);
+    }
+    acc = fmaxf(acc, 0.f);
+    return wave_sum(acc * lds[SW_W3 + lane]) + b3;
+}
+
+// Entry vertices from the entry network: the n_entry highest-scoring pivots,
+// best first, in lanes 0..n_entry-1.  Returns false (the caller then keeps the
+// reference seeds) when any score is not a number.
+template <int METRIC>
+__device__ __forceinline__ bool learned_entry(const SearchArgs& P, int qid, int lane, int& sid) {
+    const LearnedArgs& L = P.L;
+    float z[LM_P];
+#pragma unroll
+    for (int p = 0; p < LM_P; p++) {
+        float part = 0.f;
+        if constexpr (METRIC == METRIC_HAMMING) {
+            // the code as a +-1 vector of 256 components; lane l owns bits 4l..4l+3
+            const uint64_t w = ((const uint64_t*)P.queries)[(size_t)qid * 4 + (lane >> 4)];
+            const int b0 = (lane & 15) * 4;
+            const float* pr = L.proj + p * 256 + lane * 4;
+#pragma unroll
+            for (int b = 0; b < 4; b++) part = __builtin_fmaf(pr[b], ((w >> (b0 + b)) & 1ull) ? 1.f : -1.f, part);
+        } else {
+            const float* q = (const float*)P.queries + (size_t)qid * P.dstride;
+            const float* pr = L.proj + (size_t)p * P.dstride;
+            for (int e = lane; e < P.dstride; e += 64) part = __builtin_fmaf(pr[e], q[e] - L.mu[e], part);
+        }
+        z[p] = wave_sum(part) * L.proj_scale[p];
+    }
+    const float4* w1 = (const float4*)(L.e_w1 + lane * LM_P);
+    float h = L.e_b1[lane];
+#pragma unroll
+    for (int p4 = 0; p4 < LM_P / 4; p4++) {
+        const float4 w = w1[p4];
+        h = __builtin_fmaf(w.x, z[4 * p4 + 0], h);
+        h = __builtin_fmaf(w.y, z[4 * p4 + 1], h);
+        h = __builtin_fmaf(w.z, z[4 * p4 + 2], h);
+        h = __builtin_fmaf(w.w, z[4 * p4 + 3], h);
+    }
+    h = fmaxf(h, 0.f);
+    float h2 = fmaxf(dense64_global(L.e_w2, L.e_b2, h, lane), 0.f);
+    float score = dense64_global(L.e_w3, L.e_b3, h2, lane);
+    if (__ballot(!(score == score))) return false;
+    const int piv = L.pivots[lane];
+    for (int r = 0; r < P.n_entry; r++) {
+        const float m = wave_max(score);
+        const int l = ffs64(__ballot(score == m));
+        const int id = __builtin_amdgcn_readlane(piv, l);
+        if (lane == r) sid = id;
+        if (lane == l) score = -INFINITY;
+    }
+    return true;
+}
+
+// The k smallest distances seen so far, ascending in lanes 0..k-1 (k <= 64):
+// the k best of the result set (no value among them can have been evicted).
+struct TopK {
+    float v;     // this lane's entry (+inf beyond what has been seen)
+    float dk;    // wave-uniform: the k-th smallest (+inf until k are seen)
+    float d1;    // wave-uniform: the smallest
+    int k;
+    int inserts; // insertions since the last read
+
+    __device__ __forceinline__ void init(int k_) {
+        v = INFINITY;
+        dk = INFINITY;
+        d1 = INFINITY;
+        k = k_;
+        inserts = 0;
+    }
+    // Insert the distances of lanes 0..F-1 in lane order.
+    __device__ __forceinline__ void insert(float it_d, int F, int lane) {
+        uint64_t m = __ballot(lane < F && it_d < dk);
+        while (m) {
+            const int r = ffs64(m);
+            const float d = readlane_f(it_d, r);
+            const int pos = __popcll(__ballot(lane < k && v <= d));
+            const float up = __shfl(v, lane - 1 < 0 ? 0 : lane - 1, 64);
+            if (lane < k) v = lane < pos ? v : (lane == pos ? d : up);
+            dk = readlane_f(v, k - 1);
+            d1 = readlane_f(v, 0);
+            inserts++;
+            m = __ballot(lane > r && lane < F && it_d < dk);
+        }
+    }
+};
+
 // ---------------------------------------------------------------- one query
 
 // The search of query qid by the calling wave (kg_search_kernel body,
 // kernalGraph.h:192-298).  lds: this wave's bloom[W] | stage_id[64] | stage_d[64].
-template <int METRIC, int S, int G, int QCH>
-__device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int lane, unsigned long long* lds) {
+template <int METRIC, int S, int G, int QCH, int LEARN>
+__device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int lane, unsigned long long* lds,
+                                             const float* stop_lds) {
     unsigned long long* bloom = lds;
     int* stage_id = (int*)(bloom + P.bloom_words);
     float* stage_d = (float*)(stage_id + 64);
@@ -421,6 +581,13 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
     // QueryTrace, telemetry.h:5-18 (wave-uniform counters)
     int tr_seed = 0, tr_visited = 0, tr_expanded = 0, tr_slots = 0, tr_cand = 0, tr_devals = 0;
     int tr_fpush = 0, tr_rupd = 0, tr_peakf = 0, tr_peakr = 0, tr_early = 0;
+    // learned termination state (k <= 64 on this path)
+    const bool tracing = LEARN && P.L.trace != nullptr;
+    const bool stopping = LEARN && P.L.use_stop && !tracing;
+    TopK tk;
+    tk.init(k);
+    float dk_prev = INFINITY, d1_prev = INFINITY;
+    int n_evals = 0;
 
     // Bloom-test a row of ids (lane order), compute the distances of the new
     // ones and push them to both sets in order.  Returns the number of new ids.
@@ -444,6 +611,7 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
         tr_rupd += res.push_items(it_d, it_id, F, lane);
         tr_peakf = max(tr_peakf, cand.sz);
         tr_peakr = max(tr_peakr, res.sz);
+        if constexpr (LEARN) tk.insert(it_d, F, lane);
         wave_lds_sync();
         return F;
     };
@@ -453,7 +621,11 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
     {
         int sid = 0;
         bool sv = lane < P.n_entry;
-        if (sv) {
+        bool learned_seeds = false;
+        if constexpr (LEARN) {
+            if (P.L.use_entry) learned_seeds = learned_entry<METRIC>(P, qid, lane, sid);
+        }
+        if (sv && !learned_seeds) {
             if (P.entries) {
                 sid = P.entries[(size_t)qid * P.n_entry + lane];
                 sv = sid >= 0 && sid < P.n;
@@ -482,6 +654,46 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
             if (c_min > worst) {
                 tr_early = 1;
                 break;
+            }
+        }
+        if constexpr (LEARN) {
+            // Learned termination: every `interval` hops from min_hops on, the
+            // stop network sees where the search stands; trace mode records
+            // the features instead (labels come from the k-th best at the end).
+            if ((tracing || stopping) && tk.dk < INFINITY && tr_expanded >= P.L.min_hops &&
+                (tr_expanded - P.L.min_hops) % P.L.interval == 0) {
+                const float sc = P.L.dist_scale;
+                const float worst = res.sz == res.cap ? res.vmax(lane) : tk.dk;
+                float f[LM_F];
+                f[0] = (float)tr_expanded * (1.0f / 64.0f);
+                f[1] = (c_min - tk.dk) / sc;
+                f[2] = (tk.dk - tk.d1) / sc;
+                f[3] = dk_prev < INFINITY ? (dk_prev - tk.dk) / sc : 0.f;
+                f[4] = d1_prev < INFINITY ? (d1_prev - tk.d1) / sc : 0.f;
+                f[5] = (float)tk.inserts / (float)k;
+                f[6] = (float)cand.sz / (float)ef;
+                f[7] = (worst - tk.dk) / sc;
+#pragma unroll
+                for (int i = 0; i < LM_F; i++) f[i] = fminf(fmaxf(f[i], -8.f), 8.f);
+                dk_prev = tk.dk;
+                d1_prev = tk.d1;
+                tk.inserts = 0;
+                if (tracing) {
+                    if (n_evals < P.L.max_evals && lane <= LM_F) {
+                        float v = tk.dk;
+#pragma unroll
+                        for (int i = 0; i < LM_F; i++)
+                            if (lane == i) v = f[i];
+                        P.L.trace[((size_t)qid * P.L.max_evals + n_evals) * (LM_F + 1) + lane] = v;
+                    }
+                    n_evals++;
+                } else {
+                    const float logit = stop_logit(stop_lds, f, P.L.s_b3, lane);
+                    if (logit > P.L.stop_logit) {  // a NaN logit never stops
+                        tr_early = 2;
+                        break;
+                    }
+                }
             }
         }
         tr_slots += P.block;
@@ -516,6 +728,10 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
         out[i] = -1;
         if (dout) dout[i] = INFINITY;
     }
+    if (tracing && lane == 0) {
+        P.L.trace_count[qid] = min(n_evals, P.L.max_evals);
+        P.L.final_dk[qid] = tk.dk;
+    }
     if (P.trace_out && lane == 0) {
         int32_t* t = P.trace_out + (size_t)qid * 12;
         t[0] = tr_seed;
@@ -536,31 +752,56 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
 
 // ---------------------------------------------------------------- kernels
 
+// LDS of a workgroup: [stop weights when learned] then per wave
+// bloom[W] | stage_id[64] | stage_d[64].
+constexpr int STOP_LDS_U64 = (LM_STOP_FLOATS + 1) / 2;
+
 // One wave per query (the reference's one block per query, kernalGraph.h:354).
-template <int METRIC, int S, int G, int QCH>
+template <int METRIC, int S, int G, int QCH, int LEARN>
 __global__ void __launch_bounds__(256) search_kernel(SearchArgs P) {
     extern __shared__ unsigned long long smem[];
+    unsigned long long* waves_lds = smem;
+    const float* stop_lds = nullptr;
+    if constexpr (LEARN) {
+        if (P.L.use_stop && !P.L.trace) {
+            stage_stop_weights(P.L.s_weights, (float*)smem);
+            __syncthreads();
+            stop_lds = (const float*)smem;
+            waves_lds = smem + STOP_LDS_U64;
+        }
+    }
     const int wave = threadIdx.x >> 6;
     const int qid = blockIdx.x * (blockDim.x >> 6) + wave;
     if (qid >= P.nq) return;
-    search_query<METRIC, S, G, QCH>(P, qid, lane_id(), smem + (size_t)wave * (P.bloom_words + 64));
+    search_query<METRIC, S, G, QCH, LEARN>(P, qid, lane_id(), waves_lds + (size_t)wave * (P.bloom_words + 64),
+                                           stop_lds);
 }
 
 // Persistent: a fixed grid of waves drains the query queue; each wave takes
 // the next query index from a device counter until all nq are taken.  Every
 // wave leaves once the counter passes nq, so the grid always drains.
-template <int METRIC, int S, int G, int QCH>
+template <int METRIC, int S, int G, int QCH, int LEARN>
 __global__ void __launch_bounds__(256) persistent_kernel(SearchArgs P) {
     extern __shared__ unsigned long long smem[];
+    unsigned long long* waves_lds = smem;
+    const float* stop_lds = nullptr;
+    if constexpr (LEARN) {
+        if (P.L.use_stop && !P.L.trace) {
+            stage_stop_weights(P.L.s_weights, (float*)smem);
+            __syncthreads();
+            stop_lds = (const float*)smem;
+            waves_lds = smem + STOP_LDS_U64;
+        }
+    }
     const int wave = threadIdx.x >> 6;
     const int lane = lane_id();
-    unsigned long long* lds = smem + (size_t)wave * (P.bloom_words + 64);
+    unsigned long long* lds = waves_lds + (size_t)wave * (P.bloom_words + 64);
     for (;;) {
         int item = 0;
         if (lane == 0) item = atomicAdd(P.queue_head, 1);
         item = __builtin_amdgcn_readfirstlane(item);
         if (item >= P.nq) break;
-        search_query<METRIC, S, G, QCH>(P, item, lane, lds);
+        search_query<METRIC, S, G, QCH, LEARN>(P, item, lane, lds, stop_lds);
     }
 }
 
@@ -570,18 +811,29 @@ __global__ void zero_counter_kernel(int* c) {
 
 // ---------------------------------------------------------------- dispatch
 
-template <int METRIC, int S, int G, int QCH>
-static hipError_t launch_one(const SearchArgs& a, const LaunchShape& ls, hipStream_t st) {
+template <int METRIC, int S, int G, int QCH, int LEARN>
+static hipError_t launch_impl(const SearchArgs& a, const LaunchShape& ls, hipStream_t st) {
     const int threads = 64 * ls.waves_per_block;
-    const size_t lds = (size_t)ls.waves_per_block * ((size_t)a.bloom_words + 64) * 8;
+    size_t lds = (size_t)ls.waves_per_block * ((size_t)a.bloom_words + 64) * 8;
+    if (LEARN && a.L.use_stop && !a.L.trace) lds += (size_t)STOP_LDS_U64 * 8;
     if (a.queue_head) {
         hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(64), 0, st, a.queue_head);
-        hipLaunchKernelGGL((persistent_kernel<METRIC, S, G, QCH>), dim3(ls.persistent_blocks), dim3(threads), lds, st, a);
+        hipLaunchKernelGGL((persistent_kernel<METRIC, S, G, QCH, LEARN>), dim3(ls.persistent_blocks), dim3(threads),
+                           lds, st, a);
     } else {
         const int blocks = (a.nq + ls.waves_per_block - 1) / ls.waves_per_block;
-        hipLaunchKernelGGL((search_kernel<METRIC, S, G, QCH>), dim3(blocks), dim3(threads), lds, st, a);
+        hipLaunchKernelGGL((search_kernel<METRIC, S, G, QCH, LEARN>), dim3(blocks), dim3(threads), lds, st, a);
     }
     return hipGetLastError();
+}
+
+// The learned variants are built for queues up to 256 (S <= 4) and rows of at
+// most 4 chunks per lane (dim <= 1024); the host checks before launching.
+template <int METRIC, int S, int G, int QCH>
+static hipError_t launch_one(const SearchArgs& a, const LaunchShape& ls, hipStream_t st) {
+    if (!a.learned) return launch_impl<METRIC, S, G, QCH, 0>(a, ls, st);
+    if constexpr (S <= 4 && QCH <= 4) return launch_impl<METRIC, S, G, QCH, 1>(a, ls, st);
+    return hipErrorInvalidValue;
 }
 
 template <int METRIC, int S>

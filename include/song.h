@@ -232,6 +232,9 @@ int song_search_device_mixed(song_index* index, const void* d_queries, int32_t n
 int song_dev_alloc(int32_t device, int64_t bytes, void** out);
 int song_dev_free(void* p);
 int song_memcpy_h2d(void* dst, const void* src, int64_t bytes);
+/* Device-to-device copy by a plain streaming kernel (16 B per lane), so that
+ * hardware byte counters can be calibrated against a known traffic. */
+int song_dev_copy(void* dst, const void* src, int64_t bytes, double* kernel_ms_out);
 int song_memcpy_d2h(void* dst, const void* src, int64_t bytes);
 /* Copies nq host queries into a new device buffer in the index's row layout. */
 int song_queries_to_device(song_index* index, const void* queries, int32_t nq, void** d_out);
