@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -16,6 +17,7 @@
 #include "nsw_build.h"
 #include "song_gt.h"
 #include "song_kernels.h"
+#include "song_train.h"
 
 namespace {
 
@@ -68,6 +70,9 @@ struct song_index {
     int32_t* d_trace = nullptr;
     int64_t cap_q = 0, cap_ids = 0, cap_trace = 0;
     int32_t* d_counter = nullptr;  // persistent-kernel queue head
+    float* d_model = nullptr;      // learned model arrays (one allocation)
+    song::LearnedArgs model{};     // device pointers into d_model
+    bool has_model = false;
     int n_cus = 0;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     std::vector<char> stage;  // host staging for padded query rows
@@ -90,6 +95,9 @@ size_t host_row_bytes(const song_index* ix) {
 
 struct Plan {
     int ef, S, G, qch, bloom_words, bloom_shift, n_seeds, batch, persistent;
+    int learned;  // bit 0 entry, bit 1 stop
+    float stop_threshold;
+    int stop_min_hops, stop_interval;
     song::LaunchShape shape;
 };
 
@@ -139,6 +147,18 @@ int make_plan(const song_index* ix, int32_t nq, int32_t k, int32_t ef, const son
         p->G = G;
         p->qch = qch;
     }
+    p->learned = o->learned;
+    p->stop_threshold = o->stop_threshold;
+    p->stop_min_hops = o->stop_min_hops;
+    p->stop_interval = o->stop_interval ? o->stop_interval : 1;
+    if (o->learned < 0 || o->learned > 3) return fail(SONG_ERR_ARG, "learned must be 0..3");
+    if (p->stop_interval < 1 || p->stop_min_hops < 0) return fail(SONG_ERR_ARG, "stop_interval >= 1, stop_min_hops >= 0");
+    if (o->learned) {
+        if (!ix->has_model) return fail(SONG_ERR_STATE, "learned search needs a model (song_index_set_model)");
+        if (k > 64) return fail(SONG_ERR_ARG, "learned search supports k <= 64");
+        if (p->ef > 256) return fail(SONG_ERR_ARG, "learned search supports ef <= 256");
+        if (p->qch > 4) return fail(SONG_ERR_ARG, "learned search supports dim <= 1024");
+    }
     return SONG_OK;
 }
 
@@ -165,6 +185,19 @@ song::SearchArgs make_args(const song_index* ix, const Plan& p, const void* dq, 
     a.efs = nullptr;
     a.k_stride = k;
     a.queue_head = p.persistent ? ix->d_counter : nullptr;
+    a.learned = p.learned ? 1 : 0;
+    if (p.learned) {
+        a.L = ix->model;
+        a.L.use_entry = p.learned & 1;
+        a.L.use_stop = (p.learned >> 1) & 1;
+        a.L.stop_logit = p.stop_threshold;
+        a.L.min_hops = p.stop_min_hops;
+        a.L.interval = p.stop_interval;
+        a.L.trace = nullptr;
+        a.L.trace_count = nullptr;
+        a.L.final_dk = nullptr;
+        a.L.max_evals = 0;
+    }
     return a;
 }
 
@@ -381,6 +414,7 @@ void song_index_destroy(song_index* ix) {
     if (ix->d_dists) (void)hipFree(ix->d_dists);
     if (ix->d_trace) (void)hipFree(ix->d_trace);
     if (ix->d_counter) (void)hipFree(ix->d_counter);
+    if (ix->d_model) (void)hipFree(ix->d_model);
     for (auto& ev : ix->ev)
         if (ev) (void)hipEventDestroy(ev);
     if (ix->stream) (void)hipStreamDestroy(ix->stream);
@@ -522,6 +556,205 @@ int song_search_device_mixed(song_index* ix, const void* d_queries, int32_t nq, 
     tm.total_ms = ms_since(t0);
     tm.n_batches = nq > 0 ? 1 : 0;
     if (timing_out) *timing_out = tm;
+    return SONG_OK;
+}
+
+int song_index_set_model(song_index* ix, const song_learned_model* m) {
+    if (!ix || !m) return fail(SONG_ERR_ARG, "null argument");
+    const int pd = ix->metric == SONG_METRIC_HAMMING ? 256 : ix->dstride;
+    if (m->proj_dim != pd) return fail(SONG_ERR_ARG, "proj_dim must be the index row stride (256 for Hamming)");
+    if (!m->proj || !m->proj_scale || !m->entry_w1 || !m->entry_b1 || !m->entry_w2 || !m->entry_b2 ||
+        !m->entry_w3 || !m->entry_b3 || !m->pivots || !m->stop_weights || (ix->metric != SONG_METRIC_HAMMING && !m->mu))
+        return fail(SONG_ERR_ARG, "null model array");
+    if (!(m->dist_scale > 0.f)) return fail(SONG_ERR_ARG, "dist_scale must be > 0");
+    for (int c = 0; c < song::LM_C; c++)
+        if (m->pivots[c] < 0 || m->pivots[c] >= ix->n) return fail(SONG_ERR_ARG, "pivot id out of range");
+    using song::LM_C;
+    using song::LM_H;
+    using song::LM_P;
+    // one allocation: mu | proj | scale | e_w1 | e_b1 | e_w2 | e_b2 | e_w3 | e_b3 | pivots | stop
+    const size_t sizes[] = {(size_t)pd, (size_t)LM_P * pd, LM_P, LM_H * LM_P, LM_H, LM_H * LM_H, LM_H,
+                            (size_t)LM_C * LM_H, LM_C, LM_C, song::LM_STOP_FLOATS};
+    std::vector<size_t> offs;
+    size_t total = 0;
+    for (size_t sz : sizes) {
+        offs.push_back(total);
+        total += (sz + 3) / 4 * 4;  // keep every array 16-byte aligned
+    }
+    std::vector<float> host(total, 0.f);
+    auto put = [&](int i, const float* src) {
+        if (src) std::memcpy(host.data() + offs[i], src, sizes[i] * 4);
+    };
+    put(0, m->mu);
+    put(1, m->proj);
+    put(2, m->proj_scale);
+    put(3, m->entry_w1);
+    put(4, m->entry_b1);
+    put(5, m->entry_w2);
+    put(6, m->entry_b2);
+    put(7, m->entry_w3);
+    put(8, m->entry_b3);
+    std::memcpy(host.data() + offs[9], m->pivots, LM_C * 4);
+    put(10, m->stop_weights);
+    std::lock_guard<std::mutex> lock(ix->mu);
+    HIP_TRY(hipSetDevice(ix->device), "hipSetDevice");
+    if (!ix->d_model) HIP_TRY(hipMalloc(&ix->d_model, total * 4), "hipMalloc model");
+    HIP_TRY(hipMemcpy(ix->d_model, host.data(), total * 4, hipMemcpyHostToDevice), "model upload");
+    song::LearnedArgs& L = ix->model;
+    L = song::LearnedArgs{};
+    float* b = ix->d_model;
+    L.mu = b + offs[0];
+    L.proj = b + offs[1];
+    L.proj_scale = b + offs[2];
+    L.e_w1 = b + offs[3];
+    L.e_b1 = b + offs[4];
+    L.e_w2 = b + offs[5];
+    L.e_b2 = b + offs[6];
+    L.e_w3 = b + offs[7];
+    L.e_b3 = b + offs[8];
+    L.pivots = (const int32_t*)(b + offs[9]);
+    L.s_weights = b + offs[10];
+    L.s_b3 = m->stop_b3;
+    L.dist_scale = m->dist_scale;
+    ix->has_model = true;
+    return SONG_OK;
+}
+
+int song_learn_trace(song_index* ix, const void* d_queries, int32_t nq, int32_t k, int32_t ef,
+                     const song_search_opts* opts, int32_t max_evals, float* d_trace, int32_t* d_count,
+                     float* d_final_dk) {
+    if (!ix) return fail(SONG_ERR_ARG, "null index");
+    if (nq > 0 && (!d_queries || !d_trace || !d_count || !d_final_dk)) return fail(SONG_ERR_ARG, "null buffer");
+    if (max_evals < 1) return fail(SONG_ERR_ARG, "max_evals must be >= 1");
+    song_search_opts o;
+    if (opts) o = *opts;
+    else std::memset(&o, 0, sizeof o);
+    o.learned = (o.learned & 1) | 2;  // trace mode rides on the stop path
+    Plan p;
+    int rc = make_plan(ix, nq, k, ef, &o, &p);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lock(ix->mu);
+    HIP_TRY(hipSetDevice(ix->device), "hipSetDevice");
+    int32_t* d_ids = nullptr;
+    HIP_TRY(hipMalloc(&d_ids, (size_t)std::max(nq, 1) * k * 4), "hipMalloc ids");
+    hipError_t e = hipSuccess;
+    if (nq > 0) {
+        song::SearchArgs a = make_args(ix, p, d_queries, nq, k, d_ids, nullptr, nullptr);
+        a.L.trace = d_trace;
+        a.L.trace_count = d_count;
+        a.L.final_dk = d_final_dk;
+        a.L.max_evals = max_evals;
+        e = song::launch_search(a, ix->metric, p.S, p.G, p.qch, p.shape, ix->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ix->stream);
+    }
+    (void)hipFree(d_ids);
+    if (e != hipSuccess) return hip_fail(e, "trace search");
+    return SONG_OK;
+}
+
+int song_mlp_train(int32_t device, const float* X, const float* targets, int64_t n, int32_t in_dim, int32_t out_dim,
+                   int32_t epochs, int32_t batch, float lr, uint32_t seed, float* weights, float* loss_out) {
+    if (!X || !targets || !weights || n < 1 || epochs < 1 || batch < 1 || !(lr > 0.f))
+        return fail(SONG_ERR_ARG, "bad training arguments");
+    if (!((in_dim == 8 && out_dim == 1) || (in_dim == 16 && out_dim == 64)))
+        return fail(SONG_ERR_ARG, "supported shapes: 8 -> 1 (stop), 16 -> 64 (entry)");
+    if (out_dim > 1)
+        for (int64_t i = 0; i < n; i++)
+            if (!(targets[i] >= 0.f && targets[i] < (float)out_dim)) return fail(SONG_ERR_ARG, "class id out of range");
+    HIP_TRY(hipSetDevice(device), "hipSetDevice");
+    const int total = song::mlp_param_count(in_dim, out_dim);
+    const int per_wave = 8;
+    const int nwaves_max = (batch + per_wave - 1) / per_wave;
+    float *dX = nullptr, *dT = nullptr, *dW = nullptr, *dM = nullptr, *dV = nullptr, *dP = nullptr, *dL = nullptr;
+    int32_t* dO = nullptr;
+    auto cleanup = [&]() {
+        for (void* q : {(void*)dX, (void*)dT, (void*)dW, (void*)dM, (void*)dV, (void*)dP, (void*)dL, (void*)dO})
+            if (q) (void)hipFree(q);
+    };
+    hipError_t e = hipMalloc(&dX, (size_t)n * in_dim * 4);
+    if (e == hipSuccess) e = hipMalloc(&dT, (size_t)n * 4);
+    if (e == hipSuccess) e = hipMalloc(&dW, (size_t)total * 4);
+    if (e == hipSuccess) e = hipMalloc(&dM, (size_t)total * 4);
+    if (e == hipSuccess) e = hipMalloc(&dV, (size_t)total * 4);
+    if (e == hipSuccess) e = hipMalloc(&dP, (size_t)nwaves_max * (total + 1) * 4);
+    if (e == hipSuccess) e = hipMalloc(&dL, (size_t)epochs * 4);
+    if (e == hipSuccess) e = hipMalloc(&dO, (size_t)n * 4);
+    if (e == hipSuccess) e = hipMemcpy(dX, X, (size_t)n * in_dim * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dT, targets, (size_t)n * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dW, weights, (size_t)total * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(dM, 0, (size_t)total * 4);
+    if (e == hipSuccess) e = hipMemset(dV, 0, (size_t)total * 4);
+    if (e == hipSuccess) e = hipMemset(dL, 0, (size_t)epochs * 4);
+    if (e != hipSuccess) {
+        cleanup();
+        return hip_fail(e, "training setup");
+    }
+    std::vector<int32_t> order((size_t)n);
+    uint64_t rng = 0x9E3779B97F4A7C15ull ^ seed;
+    auto next = [&]() {  // splitmix64
+        uint64_t z = (rng += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    };
+    int64_t step = 0;
+    for (int ep = 0; ep < epochs && e == hipSuccess; ep++) {
+        for (int64_t i = 0; i < n; i++) order[i] = (int32_t)i;
+        for (int64_t i = n - 1; i > 0; i--) std::swap(order[i], order[next() % (uint64_t)(i + 1)]);
+        e = hipMemcpy(dO, order.data(), (size_t)n * 4, hipMemcpyHostToDevice);
+        for (int64_t b0 = 0; b0 < n && e == hipSuccess; b0 += batch) {
+            const int nb = (int)std::min<int64_t>(batch, n - b0);
+            const int nw = (nb + per_wave - 1) / per_wave;
+            step++;
+            const float bc1 = 1.f - std::pow(0.9f, (float)step), bc2 = 1.f - std::pow(0.999f, (float)step);
+            e = song::launch_mlp_grad(in_dim, out_dim, dX, dT, dO, b0, nb, per_wave, dW, dP, nw, 0);
+            if (e == hipSuccess) e = song::launch_mlp_adam(dW, dM, dV, dP, nw, total, nb, lr, bc1, bc2, dL + ep, 0);
+        }
+    }
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(weights, dW, (size_t)total * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && loss_out) {
+        e = hipMemcpy(loss_out, dL, (size_t)epochs * 4, hipMemcpyDeviceToHost);
+        for (int ep = 0; ep < epochs; ep++) loss_out[ep] /= (float)n;
+    }
+    cleanup();
+    if (e != hipSuccess) return hip_fail(e, "training");
+    return SONG_OK;
+}
+
+int song_mlp_forward(int32_t device, const float* X, int64_t n, int32_t in_dim, int32_t out_dim,
+                     const float* weights, float* logits_out) {
+    if (!X || !weights || !logits_out || n < 0) return fail(SONG_ERR_ARG, "bad arguments");
+    if (!((in_dim == 8 && out_dim == 1) || (in_dim == 16 && out_dim == 64)))
+        return fail(SONG_ERR_ARG, "supported shapes: 8 -> 1 (stop), 16 -> 64 (entry)");
+    if (n == 0) return SONG_OK;
+    HIP_TRY(hipSetDevice(device), "hipSetDevice");
+    const int total = song::mlp_param_count(in_dim, out_dim);
+    float *dX = nullptr, *dW = nullptr, *dY = nullptr;
+    hipError_t e = hipMalloc(&dX, (size_t)n * in_dim * 4);
+    if (e == hipSuccess) e = hipMalloc(&dW, (size_t)(total + 4) * 4);
+    if (e == hipSuccess) e = hipMalloc(&dY, (size_t)n * out_dim * 4);
+    if (e == hipSuccess) e = hipMemcpy(dX, X, (size_t)n * in_dim * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dW, weights, (size_t)total * 4, hipMemcpyHostToDevice);
+    song::LearnedArgs L{};
+    if (out_dim == 1) {
+        L.s_weights = dW;
+        L.s_b3 = weights[song::LM_STOP_FLOATS];
+    } else {
+        const int H = song::LM_H;
+        L.e_w1 = dW;
+        L.e_b1 = dW + H * 16;
+        L.e_w2 = L.e_b1 + H;
+        L.e_b2 = L.e_w2 + H * H;
+        L.e_w3 = L.e_b2 + H;
+        L.e_b3 = L.e_w3 + 64 * H;
+    }
+    if (e == hipSuccess) e = song::launch_learned_forward(L, out_dim == 1 ? 0 : 1, dX, n, dY, 0);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(logits_out, dY, (size_t)n * out_dim * 4, hipMemcpyDeviceToHost);
+    for (void* q : {(void*)dX, (void*)dW, (void*)dY})
+        if (q) (void)hipFree(q);
+    if (e != hipSuccess) return hip_fail(e, "mlp forward");
     return SONG_OK;
 }
 

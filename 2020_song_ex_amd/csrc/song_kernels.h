@@ -74,6 +74,10 @@ struct LaunchShape {
 
 // metric: METRIC_*; S: set slots per lane (cap <= 64*S, S in {1,2,4,8});
 // G: lanes per row (power of two); qch: 16-byte chunks per lane (G*qch*4 >= dstride).
+// which = 0: stop network on X [n x 8] -> Y [n]; 1: entry network on X [n x 16] -> Y [n x 64].
+hipError_t launch_learned_forward(const LearnedArgs& L, int which, const float* X, int64_t n, float* Y,
+                                  hipStream_t st);
+
 hipError_t launch_search(const SearchArgs& a, int metric, int S, int G, int qch, const LaunchShape& shape,
                          hipStream_t st);
 

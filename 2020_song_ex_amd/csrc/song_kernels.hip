@@ -456,6 +456,23 @@ __device__ __forceinline__ float stop_logit(const float* lds, const float (&f)[L
     return wave_sum(acc * lds[SW_W3 + lane]) + b3;
 }
 
+// Entry network on wave-uniform inputs z: lane c returns the score of pivot c.
+__device__ __forceinline__ float entry_scores(const LearnedArgs& L, const float (&z)[LM_P], int lane) {
+    const float4* w1 = (const float4*)(L.e_w1 + lane * LM_P);
+    float h = L.e_b1[lane];
+#pragma unroll
+    for (int p4 = 0; p4 < LM_P / 4; p4++) {
+        const float4 w = w1[p4];
+        h = __builtin_fmaf(w.x, z[4 * p4 + 0], h);
+        h = __builtin_fmaf(w.y, z[4 * p4 + 1], h);
+        h = __builtin_fmaf(w.z, z[4 * p4 + 2], h);
+        h = __builtin_fmaf(w.w, z[4 * p4 + 3], h);
+    }
+    h = fmaxf(h, 0.f);
+    const float h2 = fmaxf(dense64_global(L.e_w2, L.e_b2, h, lane), 0.f);
+    return dense64_global(L.e_w3, L.e_b3, h2, lane);
+}
+
 // Entry vertices from the entry network: the n_entry highest-scoring pivots,
 // best first, in lanes 0..n_entry-1.  Returns false (the caller then keeps the
 // reference seeds) when any score is not a number.
@@ -480,19 +497,7 @@ __device__ __forceinline__ bool learned_entry(const SearchArgs& P, int qid, int 
         }
         z[p] = wave_sum(part) * L.proj_scale[p];
     }
-    const float4* w1 = (const float4*)(L.e_w1 + lane * LM_P);
-    float h = L.e_b1[lane];
-#pragma unroll
-    for (int p4 = 0; p4 < LM_P / 4; p4++) {
-        const float4 w = w1[p4];
-        h = __builtin_fmaf(w.x, z[4 * p4 + 0], h);
-        h = __builtin_fmaf(w.y, z[4 * p4 + 1], h);
-        h = __builtin_fmaf(w.z, z[4 * p4 + 2], h);
-        h = __builtin_fmaf(w.w, z[4 * p4 + 3], h);
-    }
-    h = fmaxf(h, 0.f);
-    float h2 = fmaxf(dense64_global(L.e_w2, L.e_b2, h, lane), 0.f);
-    float score = dense64_global(L.e_w3, L.e_b3, h2, lane);
+    float score = entry_scores(L, z, lane);
     if (__ballot(!(score == score))) return false;
     const int piv = L.pivots[lane];
     for (int r = 0; r < P.n_entry; r++) {
@@ -807,6 +812,42 @@ __global__ void __launch_bounds__(256) persistent_kernel(SearchArgs P) {
 
 __global__ void zero_counter_kernel(int* c) {
     if (threadIdx.x == 0) *c = 0;
+}
+
+// The search kernel's inference routines on given inputs (for checking them
+// against a plain FP32 MLP): stop network (8 -> 1) or entry network (16 -> 64).
+__global__ void __launch_bounds__(64) stop_forward_kernel(LearnedArgs L, const float* __restrict__ X, int64_t n,
+                                                          float* __restrict__ Y) {
+    __shared__ float lds[LM_STOP_FLOATS];
+    stage_stop_weights(L.s_weights, lds);
+    __syncthreads();
+    const int lane = lane_id();
+    for (int64_t i = blockIdx.x; i < n; i += gridDim.x) {
+        float f[LM_F];
+#pragma unroll
+        for (int j = 0; j < LM_F; j++) f[j] = X[i * LM_F + j];
+        const float y = stop_logit(lds, f, L.s_b3, lane);
+        if (lane == 0) Y[i] = y;
+    }
+}
+
+__global__ void __launch_bounds__(64) entry_forward_kernel(LearnedArgs L, const float* __restrict__ X, int64_t n,
+                                                           float* __restrict__ Y) {
+    const int lane = lane_id();
+    for (int64_t i = blockIdx.x; i < n; i += gridDim.x) {
+        float z[LM_P];
+#pragma unroll
+        for (int j = 0; j < LM_P; j++) z[j] = X[i * LM_P + j];
+        Y[i * LM_C + lane] = entry_scores(L, z, lane);
+    }
+}
+
+hipError_t launch_learned_forward(const LearnedArgs& L, int which, const float* X, int64_t n, float* Y,
+                                  hipStream_t st) {
+    const int blocks = (int)(n < 4096 ? (n > 0 ? n : 1) : 4096);
+    if (which == 0) hipLaunchKernelGGL(stop_forward_kernel, dim3(blocks), dim3(64), 0, st, L, X, n, Y);
+    else hipLaunchKernelGGL(entry_forward_kernel, dim3(blocks), dim3(64), 0, st, L, X, n, Y);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------- dispatch

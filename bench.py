@@ -166,9 +166,19 @@ def main():
 
     abytes = algorithmic_bytes(trace, dim, block, args.k)
     achieved = abytes / (kernel_avg_ms / 1e3) / 1e9
+    slots = 1 if ef <= 64 else 2 if ef <= 128 else 4 if ef <= 256 else 8
+    kname = f"search_kernel<0,{slots},8,4,0>"
+    # HBM bytes per launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE (calibrated
+    # against a copy kernel) for this exact kernel and workload, when recorded
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        rec = json.load(open(tpath)).get(f"{kname} {args.n}x{dim} nq={nq} queue={ef} k={args.k}")
+        if rec:
+            traffic = rec["bytes_per_launch"]
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "search_kernel<L2,S=1,G=8,QCH=4>", "kernel_avg_ms": round(kernel_avg_ms, 4),
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": kname, "kernel_avg_ms": round(kernel_avg_ms, 4),
                 "algorithmic_bytes_per_launch": abytes}
 
     cpu = None

@@ -113,7 +113,14 @@ typedef struct song_search_opts {
                                 device query queue.  Same results. */
     int32_t waves_per_cu;    /* persistent grid: waves per compute unit,
                                 1..32; 0 = 32 */
-    int32_t reserved[9];
+    int32_t learned;         /* bit 0: entry vertices from the index's entry
+                                network; bit 1: stop when the stop network's
+                                logit exceeds stop_threshold.  Needs a model
+                                (song_index_set_model), k <= 64, ef <= 256. */
+    float stop_threshold;    /* logit threshold of the stop network */
+    int32_t stop_min_hops;   /* first hop at which it is consulted */
+    int32_t stop_interval;   /* consult every this many hops (>= 1) */
+    int32_t reserved[5];
 } song_search_opts;
 
 enum { SONG_MODE_PLAIN = 0, SONG_MODE_PERSISTENT = 1 };
@@ -223,6 +230,61 @@ int song_search_device_mixed(song_index* index, const void* d_queries, int32_t n
                              const int32_t* d_efs, int32_t k_max, int32_t ef_max,
                              const song_search_opts* opts, int32_t* d_ids, float* d_dists,
                              song_query_trace* d_trace, song_timing* timing_out);
+
+/* ---------------------------------------------------------------- learned entry / termination */
+
+/* A trained model for the learned search (no counterpart in the reference).
+ * Two ReLU MLPs with 2 x 64 hidden units, FP32, row-major weights:
+ *   entry: z = proj_scale * (proj (q - mu)), 16 inputs -> 64 -> 64 -> 64
+ *          scores over the 64 pivot nodes; the n_seeds best are the seeds.
+ *          For HAMMING q is the code as a +-1 vector of 256 and mu is unused.
+ *   stop:  8 per-hop features -> 64 -> 64 -> 1 logit
+ *          (stop_weights = w1 [64x8], b1 [64], w2 [64x64], b2 [64], w3 [64]).
+ * The features are defined in song_kernels.hip (search_query). */
+typedef struct song_learned_model {
+    int32_t proj_dim;        /* row stride of the index (dim rounded up to 4), or 256 */
+    const float* mu;         /* [proj_dim] or NULL (HAMMING) */
+    const float* proj;       /* [16 x proj_dim] */
+    const float* proj_scale; /* [16] */
+    const float* entry_w1;   /* [64 x 16] */
+    const float* entry_b1;   /* [64] */
+    const float* entry_w2;   /* [64 x 64] */
+    const float* entry_b2;   /* [64] */
+    const float* entry_w3;   /* [64 x 64] */
+    const float* entry_b3;   /* [64] */
+    const int32_t* pivots;   /* [64] node ids < n */
+    const float* stop_weights; /* [4800] */
+    float stop_b3;
+    float dist_scale;        /* > 0: distance differences are divided by it */
+} song_learned_model;
+
+/* Copies a model to the index's device (replacing any previous one). */
+int song_index_set_model(song_index* index, const song_learned_model* model);
+
+/* Training traces: runs the search with the stop network off and records, at
+ * every consulted hop (opts->stop_min_hops / stop_interval), the 8 features
+ * and the current k-th best distance: d_trace [nq x max_evals x 9] (device),
+ * d_count [nq] rows written, d_final_dk [nq] the k-th best at the end.
+ * Entry per opts->learned bit 0.  k <= 64, ef <= 256. */
+int song_learn_trace(song_index* index, const void* d_queries, int32_t nq, int32_t k, int32_t ef,
+                     const song_search_opts* opts, int32_t max_evals, float* d_trace, int32_t* d_count,
+                     float* d_final_dk);
+
+/* Trains a 2 x 64 ReLU MLP (in_dim -> 64 -> 64 -> out_dim) on the device with
+ * Adam.  X: host float [n x in_dim]; targets: host float [n] -- 0/1 labels
+ * (out_dim == 1, logistic loss) or class ids (out_dim > 1, softmax loss).
+ * weights: host float, w1 [64 x in] b1 [64] w2 [64x64] b2 [64] w3 [out x 64]
+ * b3 [out]; read as the starting point and overwritten with the result.
+ * loss_out (nullable): [epochs] mean training loss per epoch.
+ * Deterministic for a given seed (sample order) and input. */
+int song_mlp_train(int32_t device, const float* X, const float* targets, int64_t n, int32_t in_dim,
+                   int32_t out_dim, int32_t epochs, int32_t batch, float lr, uint32_t seed,
+                   float* weights, float* loss_out);
+
+/* The same network's forward pass on the device (the inference code the
+ * search kernel uses): logits [n x out_dim]. */
+int song_mlp_forward(int32_t device, const float* X, int64_t n, int32_t in_dim, int32_t out_dim,
+                     const float* weights, float* logits_out);
 
 /* ---------------------------------------------------------------- device memory helpers */
 
