@@ -195,6 +195,9 @@ __device__ __forceinline__ uint64_t bloom_row(unsigned long long* bloom, uint32_
 // ---------------------------------------------------------------- the two sets
 
 // Unsorted bounded set of (dist, id), entry i in slot i/64 of lane i%64.
+// Slots at or beyond sz hold NaN, which fminf / fmaxf ignore and == never
+// matches, so scans need no validity test.  The maximum is cached (wave
+// uniform) and recomputed only after it can have changed.
 template <int S>
 struct WSet {
     // ext_vector_type keeps the slots in VGPRs: a plain array gets its
@@ -203,37 +206,46 @@ struct WSet {
     typedef int ivec __attribute__((ext_vector_type(S == 1 ? 2 : S)));
     fvec d;
     ivec id;
-    int sz;  // wave-uniform
-    int cap; // wave-uniform
+    int sz;    // wave-uniform
+    int cap;   // wave-uniform
+    float mx;  // cached maximum, valid when mx_ok
+    bool mx_ok;
 
     __device__ __forceinline__ void init(int c) {
         sz = 0;
         cap = c < 1 ? 1 : c;
+        mx_ok = false;
+        mx = 0.f;
 #pragma unroll
         for (int s = 0; s < S; s++) {
-            d[s] = 0.f;
+            d[s] = __builtin_nanf("");
             id[s] = -1;
         }
     }
     __device__ __forceinline__ float vmax(int lane) const {
         float m = -INFINITY;
 #pragma unroll
-        for (int s = 0; s < S; s++)
-            if (s * 64 + lane < sz) m = fmaxf(m, d[s]);
+        for (int s = 0; s < S; s++) m = fmaxf(m, d[s]);
         return wave_max(m);
+    }
+    __device__ __forceinline__ float cmax(int lane) {
+        if (!mx_ok) {
+            mx = vmax(lane);
+            mx_ok = true;
+        }
+        return mx;
     }
     __device__ __forceinline__ float vmin(int lane) const {
         float m = INFINITY;
 #pragma unroll
-        for (int s = 0; s < S; s++)
-            if (s * 64 + lane < sz) m = fminf(m, d[s]);
+        for (int s = 0; s < S; s++) m = fminf(m, d[s]);
         return wave_min(m);
     }
     // First index whose value equals v (v is a value present in the set).
     __device__ __forceinline__ int first_eq(float v, int lane) const {
 #pragma unroll
         for (int s = 0; s < S; s++) {
-            uint64_t b = __ballot(s * 64 + lane < sz && d[s] == v);
+            uint64_t b = __ballot(d[s] == v);
             if (b) return s * 64 + ffs64(b);
         }
         return 0;
@@ -251,23 +263,25 @@ struct WSet {
         dv = readlane_f(dd, ln);
         iv = __builtin_amdgcn_readlane(ii, ln);
     }
+    // Branch-free: every lane selects, only lane idx%64 in slot idx/64 changes.
     __device__ __forceinline__ void put(int idx, float dv, int iv, int lane) {
-        if (lane == (idx & 63)) {
-            const int slot = idx >> 6;
+        const int slot = idx >> 6;
+        const bool mine = lane == (idx & 63);
 #pragma unroll
-            for (int s = 0; s < S; s++)
-                if (slot == s) {
-                    d[s] = dv;
-                    id[s] = iv;
-                }
+        for (int s = 0; s < S; s++) {
+            const bool w = mine && slot == s;
+            d[s] = w ? dv : d[s];
+            id[s] = w ? iv : id[s];
         }
     }
-    // Remove entry p, filling the hole with the last entry (gpuHeap.h:44-48).
+    // Remove entry p, filling the hole with the last entry (gpuHeap.h:44-48);
+    // the vacated last slot becomes NaN.  The caller keeps mx_ok as fits.
     __device__ __forceinline__ void remove_at(int p, int lane) {
         float ld;
         int li;
         get(sz - 1, ld, li);
         put(p, ld, li, lane);
+        put(sz - 1, __builtin_nanf(""), -1, lane);
         sz--;
     }
     // Push items it_d/it_id held by lanes 0..F-1, in lane order, with the
@@ -280,20 +294,18 @@ struct WSet {
             const int src = (lane - sz) & 63;
             const float md = __shfl(it_d, src, 64);
             const int mi = __shfl(it_id, src, 64);
-            if (src < a) {
-                const int slot = (sz + src) >> 6;
+            const int slot = src < a ? (sz + src) >> 6 : -1;
 #pragma unroll
-                for (int s = 0; s < S; s++)
-                    if (slot == s) {
-                        d[s] = md;
-                        id[s] = mi;
-                    }
+            for (int s = 0; s < S; s++) {
+                d[s] = slot == s ? md : d[s];
+                id[s] = slot == s ? mi : id[s];
             }
             sz += a;
             accepted = a;
+            mx_ok = false;
         }
         if (F > a) {
-            float M = vmax(lane);
+            float M = cmax(lane);
             int start = a;
             for (;;) {
                 const uint64_t m = __ballot(lane >= start && lane < F && it_d < M);
@@ -304,9 +316,10 @@ struct WSet {
                 const int p = first_eq(M, lane);
                 put(p, dv, iv, lane);
                 accepted++;
+                mx_ok = false;
                 start = r + 1;
                 if (start >= F) break;
-                M = vmax(lane);
+                M = cmax(lane);
             }
         }
         return accepted;
@@ -652,10 +665,11 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
         float dummy;
         int cn;
         cand.get(p, dummy, cn);
-        cand.remove_at(p, lane);
+        cand.remove_at(p, lane);  // removing the minimum leaves the maximum unless the set empties
+        if (cand.sz == 0) cand.mx_ok = false;
         tr_expanded++;
         if (res.sz == res.cap) {
-            const float worst = res.vmax(lane);
+            const float worst = res.cmax(lane);
             if (c_min > worst) {
                 tr_early = 1;
                 break;
@@ -668,7 +682,7 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
             if ((tracing || stopping) && tk.dk < INFINITY && tr_expanded >= P.L.min_hops &&
                 (tr_expanded - P.L.min_hops) % P.L.interval == 0) {
                 const float sc = P.L.dist_scale;
-                const float worst = res.sz == res.cap ? res.vmax(lane) : tk.dk;
+                const float worst = res.sz == res.cap ? res.cmax(lane) : tk.dk;
                 float f[LM_F];
                 f[0] = (float)tr_expanded * (1.0f / 64.0f);
                 f[1] = (c_min - tk.dk) / sc;
@@ -714,6 +728,7 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
         const float M = res.vmax(lane);
         res.remove_at(res.first_eq(M, lane), lane);
     }
+    res.mx_ok = false;
     const int cnt = res.sz < k ? res.sz : k;
     int32_t* out = P.ids_out + (size_t)qid * P.k_stride;
     float* dout = P.dists_out ? P.dists_out + (size_t)qid * P.k_stride : nullptr;
