@@ -37,7 +37,8 @@ class SongBatchTiming(ctypes.Structure):
 class SongSearchOpts(ctypes.Structure):
     _fields_ = [("bloom_bits", _I32), ("batch", _I32), ("n_seeds", _I32),
                 ("waves_per_block", _I32), ("lanes_per_row", _I32), ("mode", _I32),
-                ("waves_per_cu", _I32), ("reserved", _I32 * 9)]
+                ("waves_per_cu", _I32), ("learned", _I32), ("stop_threshold", ctypes.c_float),
+                ("stop_min_hops", _I32), ("stop_interval", _I32), ("reserved", _I32 * 5)]
 
 
 EXPORTS = (
@@ -46,7 +47,8 @@ EXPORTS = (
     "song_bithash_project", "song_index_create", "song_index_destroy", "song_index_row_stride",
     "song_search", "song_search_device", "song_dev_alloc", "song_dev_free", "song_memcpy_h2d",
     "song_memcpy_d2h", "song_queries_to_device", "song_index_bruteforce",
-    "song_search_device_mixed", "song_dev_copy",
+    "song_search_device_mixed", "song_dev_copy", "song_index_set_model",
+    "song_learn_trace", "song_mlp_train", "song_mlp_forward",
 )
 
 _lib = None
@@ -113,8 +115,12 @@ MODE_PLAIN, MODE_PERSISTENT = 0, 1
 
 
 def make_opts(bloom_bits=0, batch=0, n_seeds=0, waves_per_block=0, lanes_per_row=0, mode=0,
-              waves_per_cu=0):
+              waves_per_cu=0, learned=0, stop_threshold=0.0, stop_min_hops=0, stop_interval=1):
     o = SongSearchOpts()
+    o.learned = learned
+    o.stop_threshold = stop_threshold
+    o.stop_min_hops = stop_min_hops
+    o.stop_interval = stop_interval
     o.mode = mode
     o.waves_per_cu = waves_per_cu
     o.bloom_bits = bloom_bits

@@ -156,7 +156,6 @@ int make_plan(const song_index* ix, int32_t nq, int32_t k, int32_t ef, const son
     if (o->learned) {
         if (!ix->has_model) return fail(SONG_ERR_STATE, "learned search needs a model (song_index_set_model)");
         if (k > 64) return fail(SONG_ERR_ARG, "learned search supports k <= 64");
-        if (p->ef > 256) return fail(SONG_ERR_ARG, "learned search supports ef <= 256");
         if (p->qch > 4) return fail(SONG_ERR_ARG, "learned search supports dim <= 1024");
     }
     return SONG_OK;

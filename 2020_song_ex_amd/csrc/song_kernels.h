@@ -80,5 +80,9 @@ hipError_t launch_learned_forward(const LearnedArgs& L, int which, const float* 
 
 hipError_t launch_search(const SearchArgs& a, int metric, int S, int G, int qch, const LaunchShape& shape,
                          hipStream_t st);
+hipError_t launch_search_l2(const SearchArgs& a, int S, int G, int qch, const LaunchShape& shape, hipStream_t st);
+hipError_t launch_search_ip(const SearchArgs& a, int S, int G, int qch, const LaunchShape& shape, hipStream_t st);
+hipError_t launch_search_hamming(const SearchArgs& a, int S, int G, int qch, const LaunchShape& shape,
+                                 hipStream_t st);
 
 }  // namespace song

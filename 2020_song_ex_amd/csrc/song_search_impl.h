@@ -1,4 +1,7 @@
-// song_kernels.hip -- the SONG graph-search kernels for MI355X (gfx950).
+#pragma once
+// song_search_impl.h -- the SONG graph-search kernels for MI355X (gfx950).
+// Included by one translation unit per metric (song_search_l2.hip, ..._ip,
+// ..._hamming), which instantiate the launch templates for that metric.
 //
 // One 64-lane wavefront searches one query.  The algorithm is the reference's
 // kg_search_kernel / hkg_search_kernel (kernalGraph.h:184-299,
@@ -825,44 +828,8 @@ __global__ void __launch_bounds__(256) persistent_kernel(SearchArgs P) {
     }
 }
 
-__global__ void zero_counter_kernel(int* c) {
+static __global__ void zero_counter_kernel(int* c) {
     if (threadIdx.x == 0) *c = 0;
-}
-
-// The search kernel's inference routines on given inputs (for checking them
-// against a plain FP32 MLP): stop network (8 -> 1) or entry network (16 -> 64).
-__global__ void __launch_bounds__(64) stop_forward_kernel(LearnedArgs L, const float* __restrict__ X, int64_t n,
-                                                          float* __restrict__ Y) {
-    __shared__ float lds[LM_STOP_FLOATS];
-    stage_stop_weights(L.s_weights, lds);
-    __syncthreads();
-    const int lane = lane_id();
-    for (int64_t i = blockIdx.x; i < n; i += gridDim.x) {
-        float f[LM_F];
-#pragma unroll
-        for (int j = 0; j < LM_F; j++) f[j] = X[i * LM_F + j];
-        const float y = stop_logit(lds, f, L.s_b3, lane);
-        if (lane == 0) Y[i] = y;
-    }
-}
-
-__global__ void __launch_bounds__(64) entry_forward_kernel(LearnedArgs L, const float* __restrict__ X, int64_t n,
-                                                           float* __restrict__ Y) {
-    const int lane = lane_id();
-    for (int64_t i = blockIdx.x; i < n; i += gridDim.x) {
-        float z[LM_P];
-#pragma unroll
-        for (int j = 0; j < LM_P; j++) z[j] = X[i * LM_P + j];
-        Y[i * LM_C + lane] = entry_scores(L, z, lane);
-    }
-}
-
-hipError_t launch_learned_forward(const LearnedArgs& L, int which, const float* X, int64_t n, float* Y,
-                                  hipStream_t st) {
-    const int blocks = (int)(n < 4096 ? (n > 0 ? n : 1) : 4096);
-    if (which == 0) hipLaunchKernelGGL(stop_forward_kernel, dim3(blocks), dim3(64), 0, st, L, X, n, Y);
-    else hipLaunchKernelGGL(entry_forward_kernel, dim3(blocks), dim3(64), 0, st, L, X, n, Y);
-    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------- dispatch
@@ -883,12 +850,12 @@ static hipError_t launch_impl(const SearchArgs& a, const LaunchShape& ls, hipStr
     return hipGetLastError();
 }
 
-// The learned variants are built for queues up to 256 (S <= 4) and rows of at
-// most 4 chunks per lane (dim <= 1024); the host checks before launching.
+// The learned variants are built for rows of at most 4 chunks per lane
+// (dim <= 1024); the host checks before launching.
 template <int METRIC, int S, int G, int QCH>
 static hipError_t launch_one(const SearchArgs& a, const LaunchShape& ls, hipStream_t st) {
     if (!a.learned) return launch_impl<METRIC, S, G, QCH, 0>(a, ls, st);
-    if constexpr (S <= 4 && QCH <= 4) return launch_impl<METRIC, S, G, QCH, 1>(a, ls, st);
+    if constexpr (QCH <= 4) return launch_impl<METRIC, S, G, QCH, 1>(a, ls, st);
     return hipErrorInvalidValue;
 }
 
@@ -916,21 +883,12 @@ static hipError_t launch_g(const SearchArgs& a, int G, int qch, const LaunchShap
 }
 
 template <int METRIC>
-static hipError_t launch_s(const SearchArgs& a, int S, int G, int qch, const LaunchShape& wpb, hipStream_t st) {
+hipError_t launch_s(const SearchArgs& a, int S, int G, int qch, const LaunchShape& wpb, hipStream_t st) {
     switch (S) {
     case 1: return launch_g<METRIC, 1>(a, G, qch, wpb, st);
     case 2: return launch_g<METRIC, 2>(a, G, qch, wpb, st);
     case 4: return launch_g<METRIC, 4>(a, G, qch, wpb, st);
     case 8: return launch_g<METRIC, 8>(a, G, qch, wpb, st);
-    }
-    return hipErrorInvalidValue;
-}
-
-hipError_t launch_search(const SearchArgs& a, int metric, int S, int G, int qch, const LaunchShape& wpb, hipStream_t st) {
-    switch (metric) {
-    case METRIC_L2: return launch_s<METRIC_L2>(a, S, G, qch, wpb, st);
-    case METRIC_IP: return launch_s<METRIC_IP>(a, S, G, qch, wpb, st);
-    case METRIC_HAMMING: return launch_s<METRIC_HAMMING>(a, S, G, qch, wpb, st);
     }
     return hipErrorInvalidValue;
 }

@@ -25,8 +25,10 @@ def _centres_for(rng, n, n_centres):
     return rng.integers(0, n_centres, size=n)
 
 
-def gmm_int(n, nq, dim=128, n_centres=1000, sigma=25.0, seed=0):
-    """(base [n, dim], queries [nq, dim]) float32 integer-valued in [0, 255]."""
+def gmm_int(n, nq, dim=128, n_centres=1000, sigma=25.0, seed=0, n_extra=0):
+    """(base [n, dim], queries [nq, dim]) float32 integer-valued in [0, 255].
+    n_extra > 0 also returns that many further draws of the same mixture
+    (drawn after the queries, so base and queries do not change)."""
     rng = np.random.default_rng(seed)
     centres = rng.uniform(0.0, 255.0, size=(n_centres, dim)).astype(np.float32)
 
@@ -43,6 +45,8 @@ def gmm_int(n, nq, dim=128, n_centres=1000, sigma=25.0, seed=0):
 
     base = draw(n)
     queries = draw(nq)  # held-out draws from the same mixture
+    if n_extra:
+        return base, queries, draw(n_extra)
     return base, queries
 
 
@@ -133,11 +137,15 @@ def config1(seed=0):
     return {"base": base, "queries": q, "block": 16, "k": 10, "ef": 64, "bloom_bits": 1 << 15}
 
 
-def config2(seed=0, n=1_000_000, nq=10_000):
+def config2(seed=0, n=1_000_000, nq=10_000, n_extra=0):
     """1,000,000 x 128 integer GMM (1,000 centres, sigma 25), 10,000 held-out queries;
-    out-degree 32; k in {10, 100} x queue in {32, 64, 128, 256, 512}."""
-    base, q = gmm_int(n, nq, dim=128, n_centres=1000, sigma=25.0, seed=seed)
-    return {"base": base, "queries": q, "block": 32, "ks": (10, 100), "efs": (32, 64, 128, 256, 512)}
+    out-degree 32; k in {10, 100} x queue in {32, 64, 128, 256, 512}.
+    n_extra: further draws (training / validation queries for the learned search)."""
+    out = gmm_int(n, nq, dim=128, n_centres=1000, sigma=25.0, seed=seed, n_extra=n_extra)
+    cfg = {"base": out[0], "queries": out[1], "block": 32, "ks": (10, 100), "efs": (32, 64, 128, 256, 512)}
+    if n_extra:
+        cfg["extra"] = out[2]
+    return cfg
 
 
 # ------------------------------------------------------------------ file I/O

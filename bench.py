@@ -1,4 +1,4 @@
-"""Benchmark: plain wave64 SONG search on one MI355X, 1M x 128 L2 workload.
+"""Benchmark: SONG graph search on one MI355X, 1M x 128 L2 workload.
 
 Workload: 1,000,000 x 128 float32 vectors, integer values in [0, 255] from a
 1,000-centre Gaussian mixture (sigma 25), 10,000 held-out queries, NSW graph
@@ -7,8 +7,13 @@ swept (k in {10, 100}); the timed queue is the smallest reaching
 recall@10 >= 0.90 against GPU brute-force ground truth.
 
 A "step" is one search of all 10,000 queries with queries, graph and vectors
-already resident in HBM (song_search_device: one kernel launch).  value =
-queries per second over the timed steps.  The JSON line also carries the
+already resident in HBM (song_search_device: one kernel launch).  Two methods
+are timed: the plain wave64 kernel at its smallest queue reaching the target
+recall, and the learned + persistent kernel (entry and stop networks trained
+on held-out draws of the same mixture, launch and threshold autotuned on a
+validation set, then timed on the 10,000 bench queries).  value = queries per
+second of the learned + persistent method when it reaches the target recall
+on the bench queries, else of the plain method.  The JSON line also carries the
 roofline of the search kernel (algorithmic bytes from the run's own trace
 counters / HIP-event kernel time) and the reference's CPU search
 (Graph::search, graph.h:33-115, compiled from its sources into oracle/_ref)
@@ -56,13 +61,16 @@ def main():
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--nq", type=int, default=10_000)
     ap.add_argument("--k", type=int, default=10)
-    ap.add_argument("--ef", type=int, default=0, help="timed queue size (0: smallest with recall >= target)")
+    ap.add_argument("--ef", type=int, default=0, help="plain queue size (0: smallest with recall >= target)")
     ap.add_argument("--target-recall", type=float, default=0.90)
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--graph", default="", help="load/save the graph file here")
     ap.add_argument("--cpu-sample", type=int, default=4000, help="queries timed on the CPU baseline")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--train", type=int, default=20000, help="training queries for the learned search")
+    ap.add_argument("--valid", type=int, default=5000, help="validation queries for the autotuner")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run a plain copy kernel of this many MiB (byte-counter calibration)")
     args = ap.parse_args()
@@ -88,10 +96,11 @@ def main():
     threads = args.threads or min(16, os.cpu_count() or 1)
 
     t0 = time.time()
-    cfg = datasets.config2(n=args.n, nq=args.nq)
+    n_extra = 0 if args.no_learned else args.train + args.valid
+    cfg = datasets.config2(n=args.n, nq=args.nq, n_extra=n_extra)
     base, queries, block = cfg["base"], cfg["queries"], cfg["block"]
     dim = base.shape[1]
-    log(f"data {base.shape} queries {queries.shape} generated in {time.time() - t0:.1f}s")
+    log(f"data {base.shape} queries {queries.shape} (+{n_extra} held-out draws) in {time.time() - t0:.1f}s")
 
     adj = None
     if args.graph and os.path.exists(args.graph):
@@ -111,18 +120,28 @@ def main():
     kmax = 100
     d_ids = N.DeviceBuffer(nq * kmax * 4)
     d_tr = N.DeviceBuffer(nq * 48)
+    idx.search_device(dq, nq, args.k, 64, d_ids)  # first launch loads the code object
 
     t0 = time.time()
     gt, _, gt_ms = idx.bruteforce(queries, kmax)
     log(f"brute-force ground truth on the GPU in {time.time() - t0:.1f}s (kernels {gt_ms:.0f} ms)")
 
-    idx.search_device(dq, nq, args.k, 64, d_ids)  # first launch loads the code object
-
-    def run(k, ef, trace=True):
-        tm = idx.search_device(dq, nq, k, ef, d_ids, None, d_tr if trace else None)
+    def run(k, ef, trace=True, **kw):
+        tm = idx.search_device(dq, nq, k, ef, d_ids, None, d_tr if trace else None, **kw)
         ids = d_ids.download(np.empty((nq, k), np.int32))
         tr = d_tr.download(np.empty((nq, 12), np.int32)) if trace else None
         return ids, tr, tm
+
+    def timed(k, ef, **kw):
+        for _ in range(args.warmup):
+            idx.search_device(dq, nq, k, ef, d_ids, None, None, **kw)
+        kernel_ms = 0.0
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            tm = idx.search_device(dq, nq, k, ef, d_ids, None, None, **kw)
+            kernel_ms += tm["kernel_ms"]
+        wall = time.perf_counter() - t0
+        return nq * args.steps / wall, wall * 1e3 / args.steps, kernel_ms / args.steps
 
     sweep = []
     efs = (32, 64, 128, 256, 512)
@@ -132,14 +151,14 @@ def main():
                 if ef < k:
                     continue
                 ids, tr, tm = run(k, ef)
-                rec = recall_at(ids, gt, k)
-                rec10 = recall_at(ids, gt, 10) if k >= 10 else None
-                row = {"k": k, "ef": ef, "recall": round(rec, 4), "recall_at_10": rec10,
+                row = {"k": k, "ef": ef, "recall": round(recall_at(ids, gt, k), 4),
+                       "recall_at_10": round(recall_at(ids, gt, 10), 4),
                        "kernel_ms": round(tm["kernel_ms"], 3), "qps_kernel": round(nq / (tm["kernel_ms"] / 1e3)),
                        "evals_mean": float(tr[:, 5].mean()), "expanded_mean": float(tr[:, 2].mean())}
                 sweep.append(row)
                 log(f"sweep {row}")
 
+    # ---------------- plain wave64 kernel
     ef = args.ef
     if not ef:
         ef = efs[-1]
@@ -148,24 +167,13 @@ def main():
             if recall_at(ids, gt, 10) >= args.target_recall:
                 ef = e
                 break
-    ids, trace, tm = run(args.k, ef)
-    rec = recall_at(ids, gt, 10)
-    log(f"timed queue size {ef}: recall@10 {rec:.4f}")
-
-    for _ in range(args.warmup):
-        run(args.k, ef, trace=False)
-    kernel_ms = 0.0
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        tm = idx.search_device(dq, nq, args.k, ef, d_ids, None, None)
-        kernel_ms += tm["kernel_ms"]
-    wall = time.perf_counter() - t0
-    ms_per_step = wall * 1e3 / args.steps
-    qps = nq * args.steps / wall
-    kernel_avg_ms = kernel_ms / args.steps
+    ids, trace, _ = run(args.k, ef)
+    plain_rec = recall_at(ids, gt, 10)
+    plain_qps, plain_ms_step, plain_kernel_ms = timed(args.k, ef)
+    log(f"plain: queue {ef}, recall@10 {plain_rec:.4f}, {plain_qps:.0f} queries/s")
 
     abytes = algorithmic_bytes(trace, dim, block, args.k)
-    achieved = abytes / (kernel_avg_ms / 1e3) / 1e9
+    achieved = abytes / (plain_kernel_ms / 1e3) / 1e9
     slots = 1 if ef <= 64 else 2 if ef <= 128 else 4 if ef <= 256 else 8
     kname = f"search_kernel<0,{slots},8,4,0>"
     # HBM bytes per launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE (calibrated
@@ -173,14 +181,47 @@ def main():
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
-        rec = json.load(open(tpath)).get(f"{kname} {args.n}x{dim} nq={nq} queue={ef} k={args.k}")
-        if rec:
-            traffic = rec["bytes_per_launch"]
+        trec = json.load(open(tpath)).get(f"{kname} {args.n}x{dim} nq={nq} queue={ef} k={args.k}")
+        if trec:
+            traffic = trec["bytes_per_launch"]
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": kname, "kernel_avg_ms": round(kernel_avg_ms, 4),
+                "kernel": kname, "kernel_avg_ms": round(plain_kernel_ms, 4),
                 "algorithmic_bytes_per_launch": abytes}
 
+    # ---------------- learned + persistent
+    learned_out = None
+    if not args.no_learned:
+        learned = importlib.import_module("2020_song_ex_amd.learned")
+        autotune = importlib.import_module("2020_song_ex_amd.autotune")
+        extra = cfg["extra"]
+        train, valid = extra[:args.train], extra[args.train:args.train + args.valid]
+        t0 = time.time()
+        model = learned.fit(idx, base, train, k=args.k, ef=512, stop_min_hops=16, stop_interval=4,
+                            max_evals=192, entry_epochs=20, stop_epochs=4, log=log)
+        log(f"learned model trained in {time.time() - t0:.1f}s")
+        vgt, _, _ = idx.bruteforce(valid, args.k)
+        grid = {"ef": [384, 512], "learned": [3], "stop_threshold": [0.0, 1.0, 2.0, 3.0],
+                "stop_min_hops": [16], "stop_interval": [4], "mode": [1], "waves_per_cu": [16, 32],
+                "waves_per_block": [1, 4]}
+        res = autotune.tune(idx, valid, vgt, args.k, grid, args.target_recall, log=log)
+        if res.best is not None:
+            p = dict(res.best.params)
+            lef = p.pop("ef")
+            ids, ltr, _ = run(args.k, lef, **p)
+            lrec = recall_at(ids, gt, 10)
+            lqps, lms_step, lkernel_ms = timed(args.k, lef, **p)
+            learned_out = {"params": {"ef": lef, **p}, "recall_at_10": round(lrec, 4), "qps": round(lqps, 1),
+                           "ms_per_step": round(lms_step, 4), "kernel_ms": round(lkernel_ms, 4),
+                           "expanded_mean": float(ltr[:, 2].mean()), "learned_stops": float((ltr[:, 11] == 2).mean()),
+                           "speedup_vs_plain": round(lqps / plain_qps, 3),
+                           "validation_recall": round(res.best.recall, 4), "tuned": res.table()}
+            log(f"learned+persistent: {learned_out['params']} recall@10 {lrec:.4f}, {lqps:.0f} queries/s "
+                f"({lqps / plain_qps:.2f}x plain)")
+        else:
+            learned_out = {"error": "no tuned configuration reached the target recall", "tuned": res.table()}
+
+    # ---------------- CPU baseline: the reference's own Graph::search
     cpu = None
     if not args.no_cpu:
         oracle = importlib.import_module("oracle")
@@ -200,15 +241,21 @@ def main():
                "recall_at_10": round(recall_at(cids, gt[:sample], 10), 4)}
         log(f"cpu baseline {cpu}")
 
+    use_learned = (learned_out is not None and "qps" in learned_out
+                   and learned_out["recall_at_10"] >= args.target_recall)
+    value = learned_out["qps"] if use_learned else plain_qps
+    ms_step = learned_out["ms_per_step"] if use_learned else plain_ms_step
     out = {
         "metric": "QPS at recall@10=0.90, 1M×128 L2: learned+persistent vs plain; p99 latency",
-        "value": round(qps, 1), "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True,
+        "value": round(value, 1), "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"plain SONG search, {args.n}x{dim} integer GMM (1000 centres, sigma 25), "
-                               f"{nq} queries, NSW out-degree {block}, L2",
-                   "method": "plain", "k": args.k, "queue": ef, "recall_at_10": round(rec, 4),
-                   "kernel_qps": round(nq / (kernel_avg_ms / 1e3), 1)},
+        "config": {"workload": f"SONG search, {args.n}x{dim} integer GMM (1000 centres, sigma 25), "
+                               f"{nq} queries, NSW out-degree {block}, L2, k {args.k}",
+                   "method": "learned+persistent" if use_learned else "plain",
+                   "plain": {"queue": ef, "recall_at_10": round(plain_rec, 4), "qps": round(plain_qps, 1),
+                             "kernel_ms": round(plain_kernel_ms, 4)},
+                   "learned_persistent": learned_out},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "sweep": sweep,

@@ -116,7 +116,7 @@ typedef struct song_search_opts {
     int32_t learned;         /* bit 0: entry vertices from the index's entry
                                 network; bit 1: stop when the stop network's
                                 logit exceeds stop_threshold.  Needs a model
-                                (song_index_set_model), k <= 64, ef <= 256. */
+                                (song_index_set_model), k <= 64, dim <= 1024. */
     float stop_threshold;    /* logit threshold of the stop network */
     int32_t stop_min_hops;   /* first hop at which it is consulted */
     int32_t stop_interval;   /* consult every this many hops (>= 1) */
@@ -265,7 +265,7 @@ int song_index_set_model(song_index* index, const song_learned_model* model);
  * every consulted hop (opts->stop_min_hops / stop_interval), the 8 features
  * and the current k-th best distance: d_trace [nq x max_evals x 9] (device),
  * d_count [nq] rows written, d_final_dk [nq] the k-th best at the end.
- * Entry per opts->learned bit 0.  k <= 64, ef <= 256. */
+ * Entry per opts->learned bit 0.  k <= 64. */
 int song_learn_trace(song_index* index, const void* d_queries, int32_t nq, int32_t k, int32_t ef,
                      const song_search_opts* opts, int32_t max_evals, float* d_trace, int32_t* d_count,
                      float* d_final_dk);

@@ -1,0 +1,138 @@
+"""Learned entry / termination: HIP inference vs a plain FP32 MLP, HIP training,
+the plain-kernel fallback, trace mode, and the effect on work and recall."""
+import importlib
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+learned = importlib.import_module("2020_song_ex_amd.learned")
+
+
+@pytest.mark.parametrize("in_dim,out_dim", [(8, 1), (16, 64)])
+def test_kernel_inference_matches_fp32_mlp(in_dim, out_dim):
+    """The search kernel's fused MLP code against a float64-accumulated FP32 MLP (1e-5 relative)."""
+    rng = np.random.default_rng(in_dim)
+    w = learned.mlp_init(in_dim, out_dim, seed=3)
+    w += rng.normal(0, 0.05, size=w.shape).astype(np.float32)
+    X = rng.normal(size=(500, in_dim)).astype(np.float32)
+    got = learned.mlp_forward(w, X, in_dim, out_dim)
+    want = learned.mlp_forward_numpy(w, X, in_dim, out_dim)
+    np.testing.assert_allclose(got, want, rtol=1e-5, atol=1e-5 * np.abs(want).max())
+
+
+def test_training_fits_a_simple_rule():
+    """HIP training: logistic loss falls and the trained net separates a linear rule."""
+    rng = np.random.default_rng(0)
+    X = rng.normal(size=(20000, 8)).astype(np.float32)
+    y = (X[:, 0] + 0.5 * X[:, 3] > 0.2).astype(np.float32)
+    w, loss = learned.mlp_train(X, y, 8, 1, epochs=5, lr=3e-3, seed=1)
+    assert loss[-1] < loss[0] * 0.5, loss
+    pred = learned.mlp_forward(w, X, 8, 1)[:, 0] > 0
+    assert (pred == (y > 0)).mean() > 0.97
+    # deterministic for a fixed seed
+    w2, loss2 = learned.mlp_train(X, y, 8, 1, epochs=5, lr=3e-3, seed=1)
+    np.testing.assert_array_equal(w, w2)
+
+
+def test_training_softmax():
+    rng = np.random.default_rng(1)
+    centers = rng.normal(size=(64, 16)).astype(np.float32) * 3
+    lab = rng.integers(0, 64, size=20000)
+    X = (centers[lab] + rng.normal(size=(20000, 16))).astype(np.float32)
+    w, loss = learned.mlp_train(X, lab.astype(np.float32), 16, 64, epochs=8, lr=3e-3, seed=2)
+    assert loss[-1] < loss[0]
+    acc = (learned.mlp_forward(w, X, 16, 64).argmax(1) == lab).mean()
+    assert acc > 0.9, acc
+
+
+@pytest.fixture(scope="module")
+def setup(song, datasets):
+    base, q = datasets.gmm_int(20000, 400, dim=64, n_centres=40, sigma=20.0, seed=5)
+    adj = song.nsw_build(base, block=32, exact_order=False)
+    idx = song.Index(base, adj)
+    train = datasets.gmm_int(20000, 3000, dim=64, n_centres=40, sigma=20.0, seed=5)[1][200:]
+    model = learned.fit(idx, base, train, k=10, ef=128, entry_epochs=10, stop_epochs=3)
+    yield base, q, adj, idx, model
+    idx.close()
+
+
+def test_nan_model_falls_back_to_plain(song, setup):
+    """A model whose outputs are NaN leaves the plain search's results untouched."""
+    base, q, adj, idx, model = setup
+    plain = idx.search(q, 10, 128)
+    bad = learned.LearnedModel(model.proj_dim, model.mu, model.proj, model.proj_scale,
+                               np.full_like(model.entry_w, np.nan), model.pivots,
+                               np.full_like(model.stop_w, np.nan), model.dist_scale)
+    bad.set_on(idx)
+    try:
+        r = idx.search(q, 10, 128, learned=3, stop_threshold=0.0, stop_min_hops=1)
+    finally:
+        model.set_on(idx)
+    np.testing.assert_array_equal(r.ids, plain.ids)
+    np.testing.assert_array_equal(r.dists, plain.dists)
+    np.testing.assert_array_equal(r.trace, plain.trace)
+
+
+def test_stop_never_fires_equals_plain(song, setup):
+    base, q, adj, idx, model = setup
+    plain = idx.search(q, 10, 128)
+    r = idx.search(q, 10, 128, learned=2, stop_threshold=1e30, stop_min_hops=1)
+    np.testing.assert_array_equal(r.ids, plain.ids)
+    np.testing.assert_array_equal(r.trace, plain.trace)
+
+
+def test_learned_entry_matches_oracle_with_those_seeds(song, oracle, setup):
+    """With learned entry only, the search equals the oracle run from the same seed list."""
+    base, q, adj, idx, model = setup
+    r = idx.search(q, 10, 128, learned=1)
+    scores = learned.mlp_forward(model.entry_w, model.project(q, 0).astype(np.float32), 16, 64)
+    order = np.argsort(-scores, axis=1, kind="stable")[:, :4]
+    seeds = model.pivots[order]
+    oi, od, ot = oracle.search(base, adj, q, 10, 128, block=32, bloom_bits=1 << 14, group=4, entries=seeds)
+    agree = (r.ids == oi).all(1).mean()
+    # the projection is computed in a different summation order than numpy's, so
+    # a near-tie between two pivot scores can pick another seed list
+    assert agree > 0.97, agree
+
+
+def test_trace_labels(song, setup):
+    base, q, adj, idx, model = setup
+    N = song._native
+    L = learned._bind()
+    import ctypes
+    nq, maxe = q.shape[0], 64
+    dq = idx.queries_to_device(q)
+    d_tr, d_c, d_f = N.DeviceBuffer(nq * maxe * 9 * 4), N.DeviceBuffer(nq * 4), N.DeviceBuffer(nq * 4)
+    o = N.make_opts()
+    o.stop_min_hops, o.stop_interval = 4, 2
+    N.check(L.song_learn_trace(idx.handle, dq.ptr, nq, 10, 128, ctypes.byref(o), maxe, d_tr.ptr, d_c.ptr, d_f.ptr),
+            "song_learn_trace")
+    cnt = d_c.download(np.empty(nq, np.int32))
+    fin = d_f.download(np.empty(nq, np.float32))
+    tr = d_tr.download(np.empty((nq, maxe, 9), np.float32))
+    plain = idx.search(q, 10, 128)
+    # entry off: the trace run is the plain search, whose 10th distance is final
+    np.testing.assert_array_equal(fin, plain.dists[:, 9])
+    assert (cnt >= 1).all() and (cnt <= maxe).all()
+    for i in range(20):
+        dk = tr[i, :cnt[i], 8]
+        assert (np.diff(dk) <= 0).all()  # the k-th best never gets worse
+        assert dk[-1] >= fin[i]
+    assert (np.abs(tr[:, :, :8]) <= 8).all()
+
+
+def test_learned_search_saves_work(song, setup):
+    """Learned entry + stop: fewer expansions than plain at the same queue size, recall close."""
+    base, q, adj, idx, model = setup
+    gt, _, _ = idx.bruteforce(q, 10)
+
+    def rec(ids):
+        return np.mean([len(set(a) & set(b)) / 10 for a, b in zip(ids, gt)])
+
+    plain = idx.search(q, 10, 128)
+    lr = idx.search(q, 10, 128, learned=3, stop_threshold=2.0, stop_min_hops=8, stop_interval=2)
+    assert lr.trace[:, 2].mean() < plain.trace[:, 2].mean()
+    assert rec(lr.ids) > rec(plain.ids) - 0.05, (rec(lr.ids), rec(plain.ids))
+    assert (lr.trace[:, 11] == 2).any()
