@@ -48,7 +48,9 @@ EXPORTS = (
     "song_search", "song_search_device", "song_dev_alloc", "song_dev_free", "song_memcpy_h2d",
     "song_memcpy_d2h", "song_queries_to_device", "song_index_bruteforce",
     "song_search_device_mixed", "song_dev_copy", "song_index_set_model",
-    "song_learn_trace", "song_mlp_train", "song_mlp_forward",
+    "song_learn_trace", "song_mlp_train", "song_mlp_forward", "song_server_create", "song_server_submit",
+    "song_server_poll", "song_server_wait", "song_server_stats_get", "song_server_destroy",
+    "song_server_run_poisson",
 )
 
 _lib = None

@@ -17,6 +17,7 @@
 #include "nsw_build.h"
 #include "song_gt.h"
 #include "song_kernels.h"
+#include "song_server.h"
 #include "song_train.h"
 
 namespace {
@@ -243,6 +244,32 @@ const void* stage_rows(song_index* ix, const void* src, int64_t rows) {
 }
 
 }  // namespace
+
+namespace song {
+
+void set_last_error(const std::string& msg) { g_err = msg; }
+
+void index_info(const song_index* ix, int* metric, int* device) {
+    *metric = ix->metric;
+    *device = ix->device;
+}
+
+int launch_mixed_on_stream(song_index* ix, const void* d_q, int nq, const int32_t* d_ks, const int32_t* d_efs,
+                           int k_stride, int ef_max, const song_search_opts* opts, int32_t* d_ids, float* d_dists,
+                           int32_t* d_counter, hipStream_t stream) {
+    Plan p;
+    int rc = make_plan(ix, nq, k_stride, ef_max, opts, &p);
+    if (rc) return rc;
+    if (nq == 0) return SONG_OK;
+    SearchArgs a = make_args(ix, p, d_q, nq, k_stride, d_ids, d_dists, nullptr);
+    a.ks = d_ks;
+    a.efs = d_efs;
+    a.queue_head = p.persistent ? d_counter : nullptr;
+    hipError_t e = launch_search(a, ix->metric, p.S, p.G, p.qch, p.shape, stream);
+    return e == hipSuccess ? SONG_OK : hip_fail(e, "search kernel launch");
+}
+
+}  // namespace song
 
 extern "C" {
 

@@ -607,8 +607,9 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
     const bool stopping = LEARN && P.L.use_stop && !tracing;
     TopK tk;
     tk.init(k);
-    float dk_prev = INFINITY, d1_prev = INFINITY;
+    float dk_prev = INFINITY;
     int n_evals = 0;
+    int last_insert_hop = 0;
 
     // Bloom-test a row of ids (lane order), compute the distances of the new
     // ones and push them to both sets in order.  Returns the number of new ids.
@@ -632,7 +633,11 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
         tr_rupd += res.push_items(it_d, it_id, F, lane);
         tr_peakf = max(tr_peakf, cand.sz);
         tr_peakr = max(tr_peakr, res.sz);
-        if constexpr (LEARN) tk.insert(it_d, F, lane);
+        if constexpr (LEARN) {
+            const int before = tk.inserts;
+            tk.insert(it_d, F, lane);
+            if (tk.inserts != before) last_insert_hop = tr_expanded;
+        }
         wave_lds_sync();
         return F;
     };
@@ -686,20 +691,23 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
                 (tr_expanded - P.L.min_hops) % P.L.interval == 0) {
                 const float sc = P.L.dist_scale;
                 const float worst = res.sz == res.cap ? res.cmax(lane) : tk.dk;
+                int closer = 0;  // frontier entries still closer than the k-th best
+#pragma unroll
+                for (int s = 0; s < S; s++) closer += __popcll(__ballot(cand.d[s] < tk.dk));
                 float f[LM_F];
                 f[0] = (float)tr_expanded * (1.0f / 64.0f);
                 f[1] = (c_min - tk.dk) / sc;
                 f[2] = (tk.dk - tk.d1) / sc;
-                f[3] = dk_prev < INFINITY ? (dk_prev - tk.dk) / sc : 0.f;
-                f[4] = d1_prev < INFINITY ? (d1_prev - tk.d1) / sc : 0.f;
+                f[3] = (float)(tr_expanded - last_insert_hop) * (1.0f / 16.0f);
+                f[4] = (float)closer / (float)k;
                 f[5] = (float)tk.inserts / (float)k;
                 f[6] = (float)cand.sz / (float)ef;
-                f[7] = (worst - tk.dk) / sc;
+                f[7] = dk_prev < INFINITY ? (dk_prev - tk.dk) / sc : 0.f;
 #pragma unroll
                 for (int i = 0; i < LM_F; i++) f[i] = fminf(fmaxf(f[i], -8.f), 8.f);
                 dk_prev = tk.dk;
-                d1_prev = tk.d1;
                 tk.inserts = 0;
+                (void)worst;
                 if (tracing) {
                     if (n_evals < P.L.max_evals && lane <= LM_F) {
                         float v = tk.dk;

@@ -1,0 +1,408 @@
+// song_server.cpp -- continuous-batching serving loop over a resident index.
+//
+// No counterpart in the reference (its bench_gpu runs fixed batches,
+// kernalGraph.h:343-375).  Requests (one query, its k and queue size) are
+// queued by any thread; a dispatcher thread admits them into launch slots as
+// slots and device memory free up.  Each slot owns a HIP stream, pinned host
+// staging and device buffers; a launch searches its requests with per-request
+// budgets (song_search_device_mixed semantics), so requests that arrive while
+// others run on the GPU start in the next free slot and overlap on the device.
+// A request's result depends only on its query, k, queue size and the search
+// options, never on what it was batched with.
+//
+// Memory budget: every admitted request holds row_bytes + 8 + 8*k_max bytes
+// of device state until its slot completes; admission keeps the sum at or
+// under the budget (requests that do not fit wait, counted as deferred).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <random>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/song.h"
+#include "song_server.h"
+
+namespace song {
+
+using clk = std::chrono::steady_clock;
+
+struct Request {
+    int64_t ticket;
+    int32_t k, ef;
+    clk::time_point t_arrival;  // submission (or scheduled arrival) time
+    clk::time_point t_done;
+    bool done = false;
+    std::vector<char> query;
+    std::vector<int32_t> ids;
+    std::vector<float> dists;
+};
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    char* h_q = nullptr;
+    int32_t* h_ks = nullptr;
+    int32_t* h_efs = nullptr;
+    int32_t* h_ids = nullptr;
+    float* h_dists = nullptr;
+    void* d_q = nullptr;
+    int32_t* d_ks = nullptr;
+    int32_t* d_efs = nullptr;
+    int32_t* d_ids = nullptr;
+    float* d_dists = nullptr;
+    int32_t* d_counter = nullptr;
+    std::vector<Request*> reqs;
+    int64_t bytes = 0;
+    bool busy = false;
+};
+
+}  // namespace song
+
+struct song_server {
+    song_index* ix = nullptr;
+    song_server_opts opts{};
+    int device = 0;
+    size_t row_bytes = 0;
+    int64_t req_bytes = 0;
+    int capacity = 0;  // requests per slot
+    std::vector<song::Slot> slots;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<song::Request*> pending;
+    std::unordered_map<int64_t, song::Request*> all;
+    std::atomic<int64_t> next_ticket{1};
+    int64_t inflight_bytes = 0;
+    song_server_stats stats{};
+    bool stop = false;
+    std::string error;
+    std::thread dispatcher;
+};
+
+namespace {
+
+int sfail(int code, const std::string& m) {
+    song::set_last_error(m);
+    return code;
+}
+
+double ms_between(song::clk::time_point a, song::clk::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+// Launch one slot's batch (called by the dispatcher without the lock held).
+hipError_t launch_slot(song_server* sv, song::Slot& s) {
+    const int nb = (int)s.reqs.size();
+    int kmax = 1, efmax = 1;
+    for (int i = 0; i < nb; i++) {
+        song::Request* r = s.reqs[i];
+        std::memcpy(s.h_q + (size_t)i * sv->row_bytes, r->query.data(), sv->row_bytes);
+        s.h_ks[i] = r->k;
+        s.h_efs[i] = r->ef;
+        kmax = std::max(kmax, r->k);
+        efmax = std::max(efmax, std::max(r->ef, r->k));
+    }
+    const int kst = sv->opts.k_max;
+    hipError_t e = hipMemcpyAsync(s.d_q, s.h_q, (size_t)nb * sv->row_bytes, hipMemcpyHostToDevice, s.stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(s.d_ks, s.h_ks, (size_t)nb * 4, hipMemcpyHostToDevice, s.stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(s.d_efs, s.h_efs, (size_t)nb * 4, hipMemcpyHostToDevice, s.stream);
+    if (e != hipSuccess) return e;
+    int rc = song::launch_mixed_on_stream(sv->ix, s.d_q, nb, s.d_ks, s.d_efs, kst, efmax, &sv->opts.search, s.d_ids,
+                                          s.d_dists, s.d_counter, s.stream);
+    if (rc != SONG_OK) return hipErrorInvalidValue;
+    e = hipMemcpyAsync(s.h_ids, s.d_ids, (size_t)nb * kst * 4, hipMemcpyDeviceToHost, s.stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(s.h_dists, s.d_dists, (size_t)nb * kst * 4, hipMemcpyDeviceToHost, s.stream);
+    if (e == hipSuccess) e = hipEventRecord(s.done, s.stream);
+    (void)kmax;
+    return e;
+}
+
+void dispatcher_loop(song_server* sv) {
+    (void)hipSetDevice(sv->device);
+    std::unique_lock<std::mutex> lock(sv->mu);
+    for (;;) {
+        // 1. retire finished slots
+        for (auto& s : sv->slots) {
+            if (!s.busy) continue;
+            lock.unlock();
+            hipError_t q = hipEventQuery(s.done);
+            lock.lock();
+            if (q == hipErrorNotReady) continue;
+            const auto now = song::clk::now();
+            const int kst = sv->opts.k_max;
+            for (size_t i = 0; i < s.reqs.size(); i++) {
+                song::Request* r = s.reqs[i];
+                r->ids.assign(s.h_ids + i * kst, s.h_ids + i * kst + r->k);
+                r->dists.assign(s.h_dists + i * kst, s.h_dists + i * kst + r->k);
+                r->t_done = now;
+                r->done = true;
+                sv->stats.completed++;
+            }
+            if (q != hipSuccess) sv->error = std::string("search launch failed: ") + hipGetErrorString(q);
+            sv->inflight_bytes -= s.bytes;
+            s.bytes = 0;
+            s.reqs.clear();
+            s.busy = false;
+            sv->stats.in_flight = 0;
+            for (auto& t : sv->slots) sv->stats.in_flight += (int64_t)t.reqs.size();
+            sv->cv_done.notify_all();
+        }
+        // 2. admit pending requests into free slots
+        bool launched = false;
+        for (auto& s : sv->slots) {
+            if (s.busy || sv->pending.empty()) continue;
+            while (!sv->pending.empty() && (int)s.reqs.size() < sv->capacity &&
+                   sv->inflight_bytes + sv->req_bytes <= sv->opts.memory_budget) {
+                s.reqs.push_back(sv->pending.front());
+                sv->pending.pop_front();
+                s.bytes += sv->req_bytes;
+                sv->inflight_bytes += sv->req_bytes;
+            }
+            if (s.reqs.empty()) {
+                sv->stats.deferred++;
+                break;  // the budget is full: wait for a slot to retire
+            }
+            sv->stats.peak_bytes = std::max(sv->stats.peak_bytes, sv->inflight_bytes);
+            sv->stats.launches++;
+            s.busy = true;
+            sv->stats.in_flight += (int64_t)s.reqs.size();
+            lock.unlock();
+            hipError_t e = launch_slot(sv, s);
+            lock.lock();
+            if (e != hipSuccess) sv->error = std::string("launch: ") + hipGetErrorString(e);
+            launched = true;
+        }
+        bool any_busy = false;
+        for (auto& s : sv->slots) any_busy = any_busy || s.busy;
+        if (sv->stop && !any_busy && sv->pending.empty()) break;
+        if (!launched) {
+            if (any_busy) {
+                lock.unlock();
+                std::this_thread::sleep_for(std::chrono::microseconds(10));
+                lock.lock();
+            } else {
+                sv->cv_work.wait_for(lock, std::chrono::milliseconds(5),
+                                     [&] { return sv->stop || !sv->pending.empty(); });
+            }
+        }
+    }
+}
+
+void free_slot(song::Slot& s) {
+    if (s.stream) (void)hipStreamSynchronize(s.stream);
+    for (void* p : {(void*)s.h_q, (void*)s.h_ks, (void*)s.h_efs, (void*)s.h_ids, (void*)s.h_dists})
+        if (p) (void)hipHostFree(p);
+    for (void* p : {(void*)s.d_q, (void*)s.d_ks, (void*)s.d_efs, (void*)s.d_ids, (void*)s.d_dists, (void*)s.d_counter})
+        if (p) (void)hipFree(p);
+    if (s.done) (void)hipEventDestroy(s.done);
+    if (s.stream) (void)hipStreamDestroy(s.stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+int song_server_create(song_index* ix, const song_server_opts* o, song_server** out) {
+    if (!ix || !out) return sfail(SONG_ERR_ARG, "null argument");
+    *out = nullptr;
+    song_server_opts opts;
+    if (o) opts = *o;
+    else std::memset(&opts, 0, sizeof opts);
+    if (opts.max_batch <= 0) opts.max_batch = 1024;
+    if (opts.n_streams <= 0) opts.n_streams = 4;
+    if (opts.memory_budget <= 0) opts.memory_budget = (int64_t)256 << 20;
+    if (opts.k_max <= 0) opts.k_max = 100;
+    if (opts.ef_max <= 0) opts.ef_max = 512;
+    if (opts.n_streams > 16) return sfail(SONG_ERR_ARG, "n_streams must be <= 16");
+    if (opts.k_max > 512 || opts.ef_max > 512) return sfail(SONG_ERR_ARG, "k_max and ef_max must be <= 512");
+    int32_t stride = 0;
+    int rc = song_index_row_stride(ix, &stride);
+    if (rc) return rc;
+    int metric = 0, device = 0;
+    song::index_info(ix, &metric, &device);
+    song_server* sv = new song_server();
+    sv->ix = ix;
+    sv->opts = opts;
+    sv->device = device;
+    sv->row_bytes = (size_t)stride * (metric == SONG_METRIC_HAMMING ? 8 : 4);
+    sv->req_bytes = (int64_t)sv->row_bytes + 8 + 8 * (int64_t)opts.k_max;
+    sv->capacity = (int)std::max<int64_t>(1, std::min<int64_t>(opts.max_batch, opts.memory_budget / sv->req_bytes));
+    if (opts.memory_budget < sv->req_bytes) {
+        delete sv;
+        return sfail(SONG_ERR_ARG, "memory_budget is smaller than one request");
+    }
+    if (hipSetDevice(device) != hipSuccess) {
+        delete sv;
+        return sfail(SONG_ERR_HIP, "hipSetDevice");
+    }
+    sv->slots.resize(opts.n_streams);
+    const size_t cap = (size_t)sv->capacity, kst = (size_t)opts.k_max;
+    for (auto& s : sv->slots) {
+        hipError_t e = hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&s.done, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&s.h_q, cap * sv->row_bytes, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&s.h_ks, cap * 4, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&s.h_efs, cap * 4, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&s.h_ids, cap * kst * 4, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&s.h_dists, cap * kst * 4, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipMalloc(&s.d_q, cap * sv->row_bytes);
+        if (e == hipSuccess) e = hipMalloc((void**)&s.d_ks, cap * 4);
+        if (e == hipSuccess) e = hipMalloc((void**)&s.d_efs, cap * 4);
+        if (e == hipSuccess) e = hipMalloc((void**)&s.d_ids, cap * kst * 4);
+        if (e == hipSuccess) e = hipMalloc((void**)&s.d_dists, cap * kst * 4);
+        if (e == hipSuccess) e = hipMalloc((void**)&s.d_counter, 64);
+        if (e != hipSuccess) {
+            for (auto& t : sv->slots) free_slot(t);
+            delete sv;
+            return sfail(SONG_ERR_ALLOC, std::string("server buffers: ") + hipGetErrorString(e));
+        }
+    }
+    sv->dispatcher = std::thread(dispatcher_loop, sv);
+    *out = sv;
+    return SONG_OK;
+}
+
+static int submit_at(song_server* sv, const void* query, int32_t k, int32_t ef, song::clk::time_point arrival,
+                     int64_t* ticket) {
+    if (!sv || !query || !ticket) return sfail(SONG_ERR_ARG, "null argument");
+    if (k < 1 || k > sv->opts.k_max) return sfail(SONG_ERR_ARG, "k out of range for this server");
+    if (std::max(ef, k) > sv->opts.ef_max) return sfail(SONG_ERR_ARG, "queue size above the server's ef_max");
+    song::Request* r = new song::Request();
+    r->ticket = sv->next_ticket.fetch_add(1);
+    r->k = k;
+    r->ef = std::max(ef, k);
+    r->t_arrival = arrival;
+    r->query.assign((const char*)query, (const char*)query + sv->row_bytes);
+    {
+        std::lock_guard<std::mutex> lock(sv->mu);
+        if (sv->stop) {
+            delete r;
+            return sfail(SONG_ERR_STATE, "server is shutting down");
+        }
+        sv->pending.push_back(r);
+        sv->all[r->ticket] = r;
+        sv->stats.submitted++;
+    }
+    sv->cv_work.notify_one();
+    *ticket = r->ticket;
+    return SONG_OK;
+}
+
+int song_server_submit(song_server* sv, const void* query, int32_t k, int32_t ef, int64_t* ticket) {
+    return submit_at(sv, query, k, ef, song::clk::now(), ticket);
+}
+
+static int take_result(song_server* sv, song::Request* r, int32_t* ids_out, float* dists_out, double* latency_ms) {
+    if (ids_out) std::memcpy(ids_out, r->ids.data(), r->ids.size() * 4);
+    if (dists_out) std::memcpy(dists_out, r->dists.data(), r->dists.size() * 4);
+    if (latency_ms) *latency_ms = ms_between(r->t_arrival, r->t_done);
+    sv->all.erase(r->ticket);
+    delete r;
+    return 1;
+}
+
+int song_server_poll(song_server* sv, int64_t ticket, int32_t* ids_out, float* dists_out, double* latency_ms) {
+    if (!sv) return sfail(SONG_ERR_ARG, "null server");
+    std::lock_guard<std::mutex> lock(sv->mu);
+    auto it = sv->all.find(ticket);
+    if (it == sv->all.end()) return sfail(SONG_ERR_ARG, "unknown ticket");
+    if (!sv->error.empty()) return sfail(SONG_ERR_HIP, sv->error);
+    if (!it->second->done) return 0;
+    return take_result(sv, it->second, ids_out, dists_out, latency_ms);
+}
+
+int song_server_wait(song_server* sv, int64_t ticket, int32_t timeout_ms, int32_t* ids_out, float* dists_out,
+                     double* latency_ms) {
+    if (!sv) return sfail(SONG_ERR_ARG, "null server");
+    std::unique_lock<std::mutex> lock(sv->mu);
+    auto it = sv->all.find(ticket);
+    if (it == sv->all.end()) return sfail(SONG_ERR_ARG, "unknown ticket");
+    song::Request* r = it->second;
+    const bool ok = sv->cv_done.wait_for(lock, std::chrono::milliseconds(timeout_ms < 0 ? 0 : timeout_ms),
+                                         [&] { return r->done || !sv->error.empty(); });
+    if (!sv->error.empty()) return sfail(SONG_ERR_HIP, sv->error);
+    if (!ok) return 0;
+    return take_result(sv, r, ids_out, dists_out, latency_ms);
+}
+
+int song_server_stats_get(song_server* sv, song_server_stats* out) {
+    if (!sv || !out) return sfail(SONG_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> lock(sv->mu);
+    *out = sv->stats;
+    out->memory_budget = sv->opts.memory_budget;
+    out->slot_capacity = sv->capacity;
+    return SONG_OK;
+}
+
+void song_server_destroy(song_server* sv) {
+    if (!sv) return;
+    {
+        std::lock_guard<std::mutex> lock(sv->mu);
+        sv->stop = true;
+    }
+    sv->cv_work.notify_all();
+    if (sv->dispatcher.joinable()) sv->dispatcher.join();
+    for (auto& s : sv->slots) free_slot(s);
+    for (auto& kv : sv->all) delete kv.second;
+    for (auto* r : sv->pending)
+        if (!sv->all.count(r->ticket)) delete r;
+    delete sv;
+}
+
+int song_server_run_poisson(song_server* sv, const void* queries, int32_t nq, const int32_t* ks, const int32_t* efs,
+                            double rate_qps, uint64_t seed, int32_t* ids_out, float* dists_out, double* latency_ms_out,
+                            double* wall_ms_out) {
+    if (!sv || (nq > 0 && (!queries || !ks || !efs || !latency_ms_out)) || nq < 0 || !(rate_qps > 0))
+        return sfail(SONG_ERR_ARG, "bad arguments");
+    std::mt19937_64 rng(seed);
+    std::exponential_distribution<double> gap(rate_qps / 1e6);  // microseconds
+    std::vector<int64_t> tickets((size_t)nq);
+    std::vector<song::clk::time_point> sched((size_t)nq);
+    const size_t hb = sv->row_bytes;  // callers pass queries in the index row layout
+    const auto t0 = song::clk::now();
+    double t_us = 0;
+    for (int i = 0; i < nq; i++) {
+        t_us += gap(rng);
+        sched[i] = t0 + std::chrono::microseconds((int64_t)t_us);
+    }
+    for (int i = 0; i < nq; i++) {
+        while (song::clk::now() < sched[i]) {
+            const auto left = sched[i] - song::clk::now();
+            if (left > std::chrono::microseconds(200)) std::this_thread::sleep_for(left / 2);
+        }
+        // latency counts from the scheduled arrival, not from when the generator got to it
+        int rc = submit_at(sv, (const char*)queries + (size_t)i * hb, ks[i], efs[i], sched[i], &tickets[i]);
+        if (rc) return rc;
+    }
+    const int kst = sv->opts.k_max;
+    auto t_last = t0;
+    for (int i = 0; i < nq; i++) {
+        std::vector<int32_t> ids(ks[i]);
+        std::vector<float> d(ks[i]);
+        int rc = song_server_wait(sv, tickets[i], 60000, ids.data(), d.data(), &latency_ms_out[i]);
+        if (rc < 0) return rc;
+        if (rc == 0) return sfail(SONG_ERR_STATE, "request timed out");
+        if (ids_out) {
+            std::fill(ids_out + (size_t)i * kst, ids_out + (size_t)(i + 1) * kst, -1);
+            std::copy(ids.begin(), ids.end(), ids_out + (size_t)i * kst);
+        }
+        if (dists_out) {
+            std::fill(dists_out + (size_t)i * kst, dists_out + (size_t)(i + 1) * kst, INFINITY);
+            std::copy(d.begin(), d.end(), dists_out + (size_t)i * kst);
+        }
+        const auto done = sched[i] + std::chrono::microseconds((int64_t)(latency_ms_out[i] * 1000.0));
+        if (done > t_last) t_last = done;
+    }
+    if (wall_ms_out) *wall_ms_out = ms_between(t0, t_last);
+    return SONG_OK;
+}
+
+}  // extern "C"

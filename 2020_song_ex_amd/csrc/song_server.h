@@ -1,0 +1,20 @@
+// song_server.h -- internal hooks the serving loop uses from the index runtime.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/song.h"
+
+namespace song {
+
+void set_last_error(const std::string& msg);
+void index_info(const song_index* ix, int* metric, int* device);
+// Per-request budgets, launched on `stream` with the caller's queue counter
+// (persistent mode); no synchronisation.
+int launch_mixed_on_stream(song_index* ix, const void* d_q, int nq, const int32_t* d_ks, const int32_t* d_efs,
+                           int k_stride, int ef_max, const song_search_opts* opts, int32_t* d_ids, float* d_dists,
+                           int32_t* d_counter, hipStream_t stream);
+
+}  // namespace song

@@ -3,14 +3,17 @@
 Two 2 x 64 ReLU MLPs (FP32), trained on the device by libsong's HIP trainer
 (song_mlp_train) and run fused inside the search kernel:
 
-* entry network: a 16-dimensional projection of the query -> scores over 64
+* entry network: the query's 16 leading principal components (of a base
+  sample, standardised) -> scores over 64
   pivot nodes (nodes nearest to k-means centroids of the base); the best
   n_seeds pivots replace the reference's evenly spaced seeds
   (kernalGraph.h:215-216).  Label: the pivot nearest to the query.
-* stop network: 8 features of where the search stands (hop count, popped
-  distance vs the k-th best, recent improvements, frontier fill, ...; see
-  song_kernels.hip) -> probability that the k-th best distance is already
-  final.  Labels come from traces of the search itself (song_learn_trace):
+* stop network: 8 features of where the search stands -- hop count, popped
+  distance minus the k-th best, spread of the top k, hops since the top k last
+  changed, frontier entries still closer than the k-th best, recent top-k
+  insertions, frontier fill, recent improvement of the k-th best (distances
+  divided by a per-index scale; song_search_impl.h) -> probability that the
+  k-th best distance is already final.  Labels come from traces of the search itself (song_learn_trace):
   at a consulted hop t the label is 1 when the k-th best distance at t equals
   the one at the end of the search.
 
@@ -221,7 +224,13 @@ def fit(index: Index, base, train_queries, *, k=10, ef=256, n_seeds=4, bloom_bit
         mu = np.zeros(proj_dim, np.float32)
         sample = base[rng.choice(base.shape[0], size=min(100_000, base.shape[0]), replace=False)]
         mu[:base.shape[1]] = sample.mean(0)
-    proj = rng.normal(size=(P_DIM, proj_dim)).astype(np.float32)
+    # projection: the 16 leading principal directions of a base sample
+    sample = _as_rows(base[rng.choice(base.shape[0], size=min(50_000, base.shape[0]), replace=False)],
+                      metric, proj_dim).astype(np.float64)
+    sample -= mu
+    cov = sample.T @ sample / sample.shape[0]
+    evals, evecs = np.linalg.eigh(cov)
+    proj = evecs[:, ::-1][:, :P_DIM].T.astype(np.float32)
     if metric != N.METRIC_HAMMING:
         proj[:, base.shape[1]:] = 0.0
     model = LearnedModel(proj_dim, mu, proj, np.ones(P_DIM, np.float32), mlp_init(P_DIM, N_PIVOTS, seed),

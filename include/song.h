@@ -286,6 +286,58 @@ int song_mlp_train(int32_t device, const float* X, const float* targets, int64_t
 int song_mlp_forward(int32_t device, const float* X, int64_t n, int32_t in_dim, int32_t out_dim,
                      const float* weights, float* logits_out);
 
+/* ---------------------------------------------------------------- serving */
+
+/* Continuous-batching server over a resident index (no counterpart in the
+ * reference, whose bench_gpu runs fixed batches, kernalGraph.h:343-375).
+ * Requests queue from any thread; a dispatcher admits them into n_streams
+ * launch slots as slots and device memory free up, so requests that arrive
+ * while others run start in the next free slot and overlap on the device.
+ * Each request holds row_bytes + 8 + 8*k_max bytes of device state while in
+ * flight; admission keeps the total within memory_budget.  A request's result
+ * depends only on its query, k, queue size and `search` options. */
+typedef struct song_server song_server;
+
+typedef struct song_server_opts {
+    int32_t max_batch;        /* requests per launch, 0 = 1024 */
+    int32_t n_streams;        /* launch slots (HIP streams), 1..16, 0 = 4 */
+    int64_t memory_budget;    /* bytes of in-flight request state, 0 = 256 MiB */
+    int32_t k_max;            /* largest k accepted, 0 = 100 */
+    int32_t ef_max;           /* largest queue size accepted (<= 512), 0 = 512 */
+    song_search_opts search;  /* options of every launch (mode, learned, ...) */
+} song_server_opts;
+
+typedef struct song_server_stats {
+    int64_t submitted;
+    int64_t completed;
+    int64_t in_flight;
+    int64_t launches;
+    int64_t deferred;         /* admission passes that found the budget full */
+    int64_t peak_bytes;       /* largest in-flight request state seen */
+    int64_t memory_budget;
+    int32_t slot_capacity;    /* requests per launch after the budget */
+    int32_t pad_;
+} song_server_stats;
+
+int song_server_create(song_index* index, const song_server_opts* opts, song_server** out);
+/* query: one row in the index's device row layout (song_index_row_stride). */
+int song_server_submit(song_server* server, const void* query, int32_t k, int32_t ef, int64_t* ticket);
+/* 1 = done (results copied, ticket released), 0 = pending, < 0 = error.
+ * latency_ms: completion time minus arrival. */
+int song_server_poll(song_server* server, int64_t ticket, int32_t* ids_out, float* dists_out,
+                     double* latency_ms);
+int song_server_wait(song_server* server, int64_t ticket, int32_t timeout_ms, int32_t* ids_out,
+                     float* dists_out, double* latency_ms);
+int song_server_stats_get(song_server* server, song_server_stats* out);
+void song_server_destroy(song_server* server);
+/* Load generator: submits nq requests at Poisson arrival times (rate_qps,
+ * seeded), waits for all of them and reports each latency from its scheduled
+ * arrival; ids_out / dists_out: [nq x k_max], -1 / inf padded; wall_ms_out:
+ * first arrival to last completion. */
+int song_server_run_poisson(song_server* server, const void* queries, int32_t nq, const int32_t* ks,
+                            const int32_t* efs, double rate_qps, uint64_t seed, int32_t* ids_out,
+                            float* dists_out, double* latency_ms_out, double* wall_ms_out);
+
 /* ---------------------------------------------------------------- device memory helpers */
 
 /* Plain device buffers for callers without a GPU runtime of their own

@@ -116,11 +116,12 @@ def test_trace_labels(song, setup):
     # entry off: the trace run is the plain search, whose 10th distance is final
     np.testing.assert_array_equal(fin, plain.dists[:, 9])
     assert (cnt >= 1).all() and (cnt <= maxe).all()
-    for i in range(20):
-        dk = tr[i, :cnt[i], 8]
+    for i in range(nq):
+        rows = tr[i, :cnt[i]]
+        dk = rows[:, 8]
         assert (np.diff(dk) <= 0).all()  # the k-th best never gets worse
         assert dk[-1] >= fin[i]
-    assert (np.abs(tr[:, :, :8]) <= 8).all()
+        assert (np.abs(rows[:, :8]) <= 8).all()  # features are clamped to [-8, 8]
 
 
 def test_learned_search_saves_work(song, setup):
