@@ -1,0 +1,74 @@
+"""Checks against data the reference itself produced (tests/golden/, written by
+make_fixtures.py from its published SIFT1M runs).
+
+* The trace relations the oracle and the kernel keep hold on every reference
+  per-query trace (10 files x 1,000 queries, raw and Hamming kernels).
+* The CSV schemas the bench writes and the driver reads are the reference's.
+"""
+import importlib
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def traces():
+    z = np.load(os.path.join(GOLDEN, "reference_query_traces.npz"))
+    return {k: z[k] for k in z.files}
+
+
+@pytest.mark.parametrize("name", sorted(traces()))
+def test_reference_trace_relations(name):
+    tr = traces()[name]
+    block = 32  # BLOCK_SIZE, config.h:9
+    pq = int(name[-4:])
+    assert tr.shape == (1000, 12)
+    assert (tr[:, 1] == tr[:, 4] + tr[:, 0]).all()           # visited = candidate + seed
+    assert (tr[:, 5] == tr[:, 1]).all()                      # distance_evals = visited
+    assert (tr[:, 3] == block * (tr[:, 2] - tr[:, 11])).all()  # slots = 32 (expanded - early)
+    assert (tr[:, 0] == 4).all()                             # four seeds
+    assert (tr[:, 8] <= pq).all() and (tr[:, 9] <= pq).all()  # peaks bounded by the queue
+    assert (tr[:, 10] == 10).all()
+
+
+def test_oracle_traces_keep_the_same_relations_at_reference_scale(oracle, song, datasets):
+    """The oracle on a SIFT-like graph: the same relations, and similar magnitudes
+    per expansion as the reference's pq=50 run (neighbour slots per expansion = 32)."""
+    base, q = datasets.gmm_int(30000, 200, dim=128, n_centres=100, sigma=25.0, seed=2)
+    adj = song.nsw_build(base, block=32, n_threads=4, batch_div=64, seq_prefix=1024, exact_order=False)
+    _, _, tr = oracle.search(base, adj, q, 10, 50, block=32, group=8)
+    ref = traces()["repro_gpu_raw_pq0050"]
+    assert (tr[:, 1] == tr[:, 4] + tr[:, 0]).all()
+    assert (tr[:, 3] == 32 * (tr[:, 2] - tr[:, 11])).all()
+    # new neighbours per expansion: same order as the reference's run
+    mine = tr[:, 4].sum() / tr[:, 2].sum()
+    theirs = ref[:, 4].sum() / ref[:, 2].sum()
+    assert 0.3 * theirs < mine < 3 * theirs, (mine, theirs)
+
+
+def test_sweep_schema_is_the_references():
+    heads = json.load(open(os.path.join(GOLDEN, "reference_csv_headers.json")))
+    rc = importlib.import_module("2020_song_ex_amd.run_compare")
+    for col in rc.SWEEP_REQUIRED:
+        assert col in heads["sweep_summary"]
+    src = open(os.path.join(os.path.dirname(GOLDEN), "..", "2020_song_ex_amd", "csrc", "bench_gpu.cpp")).read()
+    flat = "".join(re.findall(r'"((?:[^"\\]|\\.)*)"', src))  # adjacent C string literals joined
+    for key in ("sweep_summary", "batch_stats", "query_stats"):
+        assert ",".join(heads[key]) in flat, key
+
+
+def test_driver_reads_reference_shaped_summary(tmp_path):
+    heads = json.load(open(os.path.join(GOLDEN, "reference_csv_headers.json")))["sweep_summary"]
+    rc = importlib.import_module("2020_song_ex_amd.run_compare")
+    row = {h: "0" for h in heads}
+    row.update({"method": "repro_gpu_raw", "pq": "50", "build_ms": "1.5", "search_ms_mean": "8.87",
+                "qps_mean": "112736", "recall": "0.8838"})
+    p = tmp_path / "sweep_summary.csv"
+    with open(p, "w") as f:
+        f.write(",".join(heads) + "\n" + ",".join(row[h] for h in heads) + "\n")
+    rows = rc.load_sweep_rows(p, 10)
+    assert rows[0]["qps"] == 112736 and rows[0]["recall"] == 0.8838 and rows[0]["pq"] == 50
