@@ -72,8 +72,13 @@ struct song_index {
     int64_t cap_q = 0, cap_ids = 0, cap_trace = 0;
     int32_t* d_counter = nullptr;  // persistent-kernel queue head
     float* d_model = nullptr;      // learned model arrays (one allocation)
-    song::LearnedArgs model{};     // device pointers into d_model
+    size_t model_floats = 0;
+    song::PlanArgs plan{};         // query-side networks: device pointers into d_model
+    song::LearnedArgs model{};     // stop network: device pointers into d_model
     bool has_model = false;
+    bool has_cost = false;
+    int32_t* d_plan = nullptr;     // planning-pass workspace: entries | cost | order
+    int64_t cap_plan = 0;
     int n_cus = 0;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     std::vector<char> stage;  // host staging for padded query rows
@@ -96,7 +101,7 @@ size_t host_row_bytes(const song_index* ix) {
 
 struct Plan {
     int ef, S, G, qch, bloom_words, bloom_shift, n_seeds, batch, persistent;
-    int learned;  // bit 0 entry, bit 1 stop
+    int learned;  // bit 0 entry, bit 1 stop, bit 2 cost-ordered queue
     float stop_threshold;
     int stop_min_hops, stop_interval;
     song::LaunchShape shape;
@@ -111,7 +116,7 @@ int make_plan(const song_index* ix, int32_t nq, int32_t k, int32_t ef, const son
     if (ef < k) ef = k;  // kernalGraph.h:315
     if (ef > 512) return fail(SONG_ERR_ARG, "ef (queue size) must be <= 512 after ef = max(ef, k)");
     p->ef = ef;
-    p->S = ef <= 64 ? 1 : ef <= 128 ? 2 : ef <= 256 ? 4 : 8;
+    p->S = ef <= 64 ? 1 : ef <= 128 ? 2 : ef <= 192 ? 3 : ef <= 256 ? 4 : ef <= 384 ? 6 : 8;
     int64_t bits = o->bloom_bits ? o->bloom_bits : 16384;
     if (!is_pow2(bits) || bits < 512 || bits > 65536)
         return fail(SONG_ERR_ARG, "bloom_bits must be a power of two in [512, 65536]");
@@ -152,12 +157,13 @@ int make_plan(const song_index* ix, int32_t nq, int32_t k, int32_t ef, const son
     p->stop_threshold = o->stop_threshold;
     p->stop_min_hops = o->stop_min_hops;
     p->stop_interval = o->stop_interval ? o->stop_interval : 1;
-    if (o->learned < 0 || o->learned > 3) return fail(SONG_ERR_ARG, "learned must be 0..3");
+    if (o->learned < 0 || o->learned > 7) return fail(SONG_ERR_ARG, "learned must be 0..7");
     if (p->stop_interval < 1 || p->stop_min_hops < 0) return fail(SONG_ERR_ARG, "stop_interval >= 1, stop_min_hops >= 0");
     if (o->learned) {
         if (!ix->has_model) return fail(SONG_ERR_STATE, "learned search needs a model (song_index_set_model)");
         if (k > 64) return fail(SONG_ERR_ARG, "learned search supports k <= 64");
         if (p->qch > 4) return fail(SONG_ERR_ARG, "learned search supports dim <= 1024");
+        if ((o->learned & 4) && !ix->has_cost) return fail(SONG_ERR_STATE, "the model has no cost network");
     }
     return SONG_OK;
 }
@@ -185,11 +191,11 @@ song::SearchArgs make_args(const song_index* ix, const Plan& p, const void* dq, 
     a.efs = nullptr;
     a.k_stride = k;
     a.queue_head = p.persistent ? ix->d_counter : nullptr;
-    a.learned = p.learned ? 1 : 0;
-    if (p.learned) {
+    a.order = nullptr;
+    a.learned = (p.learned & 2) ? 1 : 0;
+    if (a.learned) {
         a.L = ix->model;
-        a.L.use_entry = p.learned & 1;
-        a.L.use_stop = (p.learned >> 1) & 1;
+        a.L.use_stop = 1;
         a.L.stop_logit = p.stop_threshold;
         a.L.min_hops = p.stop_min_hops;
         a.L.interval = p.stop_interval;
@@ -243,6 +249,40 @@ const void* stage_rows(song_index* ix, const void* src, int64_t rows) {
     return ix->stage.data();
 }
 
+// int32 words of planning workspace for nq queries: entries [nq x n_seeds],
+// cost [nq], order [nq].
+int64_t plan_words(int64_t nq, int n_seeds) { return nq * (n_seeds + 2); }
+
+bool needs_plan(const Plan& p) { return (p.learned & 5) != 0; }
+
+int ensure_plan_ws(song_index* ix, int64_t nq, int n_seeds) {
+    const int64_t need = plan_words(nq, n_seeds);
+    if (need <= ix->cap_plan) return SONG_OK;
+    if (ix->d_plan) (void)hipFree(ix->d_plan);
+    ix->d_plan = nullptr;
+    ix->cap_plan = 0;
+    HIP_TRY(hipMalloc(&ix->d_plan, need * 4), "hipMalloc planning workspace");
+    ix->cap_plan = need;
+    return SONG_OK;
+}
+
+// The planning pass of a learned launch (entry seeds and / or the cost
+// order) into ws; points the launch at its outputs.
+hipError_t run_plan(const song_index* ix, const Plan& p, song::SearchArgs& a, int32_t* ws, hipStream_t st) {
+    if (!needs_plan(p) || a.nq <= 0) return hipSuccess;
+    song::PlanArgs P = ix->plan;
+    P.queries = a.queries;
+    P.nq = a.nq;
+    P.n_entry = p.n_seeds;
+    P.entries = (p.learned & 1) ? ws : nullptr;
+    P.cost = (p.learned & 4) ? (float*)(ws + (size_t)a.nq * p.n_seeds) : nullptr;
+    int32_t* order = (p.learned & 4) ? ws + (size_t)a.nq * (p.n_seeds + 1) : nullptr;
+    hipError_t e = song::launch_plan(P, ix->metric, order, st);
+    a.entries = P.entries;
+    a.order = order;
+    return e;
+}
+
 }  // namespace
 
 namespace song {
@@ -256,7 +296,7 @@ void index_info(const song_index* ix, int* metric, int* device) {
 
 int launch_mixed_on_stream(song_index* ix, const void* d_q, int nq, const int32_t* d_ks, const int32_t* d_efs,
                            int k_stride, int ef_max, const song_search_opts* opts, int32_t* d_ids, float* d_dists,
-                           int32_t* d_counter, hipStream_t stream) {
+                           int32_t* d_counter, int32_t* d_plan_ws, hipStream_t stream) {
     Plan p;
     int rc = make_plan(ix, nq, k_stride, ef_max, opts, &p);
     if (rc) return rc;
@@ -265,9 +305,16 @@ int launch_mixed_on_stream(song_index* ix, const void* d_q, int nq, const int32_
     a.ks = d_ks;
     a.efs = d_efs;
     a.queue_head = p.persistent ? d_counter : nullptr;
+    if (needs_plan(p)) {
+        if (!d_plan_ws) return fail(SONG_ERR_ARG, "learned entry / order needs a planning workspace");
+        hipError_t e = run_plan(ix, p, a, d_plan_ws, stream);
+        if (e != hipSuccess) return hip_fail(e, "planning pass");
+    }
     hipError_t e = launch_search(a, ix->metric, p.S, p.G, p.qch, p.shape, stream);
     return e == hipSuccess ? SONG_OK : hip_fail(e, "search kernel launch");
 }
+
+int64_t plan_ws_words(int64_t nq) { return plan_words(nq, 64); }
 
 }  // namespace song
 
@@ -441,6 +488,7 @@ void song_index_destroy(song_index* ix) {
     if (ix->d_trace) (void)hipFree(ix->d_trace);
     if (ix->d_counter) (void)hipFree(ix->d_counter);
     if (ix->d_model) (void)hipFree(ix->d_model);
+    if (ix->d_plan) (void)hipFree(ix->d_plan);
     for (auto& ev : ix->ev)
         if (ev) (void)hipEventDestroy(ev);
     if (ix->stream) (void)hipStreamDestroy(ix->stream);
@@ -469,6 +517,7 @@ int song_search(song_index* ix, const void* queries, int32_t nq, int32_t k, int3
     const int64_t bmax = std::min<int64_t>(p.batch, std::max<int32_t>(nq, 1));
     rc = ensure_ws(ix, bmax, k, dists_out != nullptr, trace_out != nullptr, &tm.alloc_ms);
     if (rc) return rc;
+    if (needs_plan(p) && (rc = ensure_plan_ws(ix, bmax, p.n_seeds))) return rc;
     const size_t hb = host_row_bytes(ix), rb = row_bytes(ix);
     auto t0 = std::chrono::steady_clock::now();
     int bi = 0;
@@ -480,6 +529,7 @@ int song_search(song_index* ix, const void* queries, int32_t nq, int32_t k, int3
         HIP_TRY(hipEventRecord(ix->ev[1], ix->stream), "hipEventRecord");
         song::SearchArgs a = make_args(ix, p, ix->d_q, bs, k, ix->d_ids, dists_out ? ix->d_dists : nullptr,
                                        trace_out ? ix->d_trace : nullptr);
+        HIP_TRY(run_plan(ix, p, a, ix->d_plan, ix->stream), "planning pass");
         HIP_TRY(song::launch_search(a, ix->metric, p.S, p.G, p.qch, p.shape, ix->stream), "search kernel launch");
         HIP_TRY(hipEventRecord(ix->ev[2], ix->stream), "hipEventRecord");
         HIP_TRY(hipMemcpyAsync(ids_out + off * k, ix->d_ids, (size_t)bs * k * 4, hipMemcpyDeviceToHost, ix->stream),
@@ -532,6 +582,8 @@ int song_search_device(song_index* ix, const void* d_queries, int32_t nq, int32_
     std::memset(&tm, 0, sizeof tm);
     tm.ef_used = p.ef;
     const size_t rb = row_bytes(ix);
+    if (needs_plan(p) && (rc = ensure_plan_ws(ix, std::min<int64_t>(p.batch, std::max<int32_t>(nq, 1)), p.n_seeds)))
+        return rc;
     auto t0 = std::chrono::steady_clock::now();
     HIP_TRY(hipEventRecord(ix->ev[1], ix->stream), "hipEventRecord");
     int bi = 0;
@@ -540,6 +592,7 @@ int song_search_device(song_index* ix, const void* d_queries, int32_t nq, int32_
         song::SearchArgs a = make_args(ix, p, (const char*)d_queries + off * rb, bs, k, d_ids + off * k,
                                        d_dists ? d_dists + off * k : nullptr,
                                        d_trace ? (int32_t*)(d_trace + off) : nullptr);
+        HIP_TRY(run_plan(ix, p, a, ix->d_plan, ix->stream), "planning pass");
         HIP_TRY(song::launch_search(a, ix->metric, p.S, p.G, p.qch, p.shape, ix->stream), "search kernel launch");
     }
     HIP_TRY(hipEventRecord(ix->ev[2], ix->stream), "hipEventRecord");
@@ -566,12 +619,14 @@ int song_search_device_mixed(song_index* ix, const void* d_queries, int32_t nq, 
     song_timing tm;
     std::memset(&tm, 0, sizeof tm);
     tm.ef_used = p.ef;
+    if (needs_plan(p) && (rc = ensure_plan_ws(ix, std::max<int32_t>(nq, 1), p.n_seeds))) return rc;
     auto t0 = std::chrono::steady_clock::now();
     HIP_TRY(hipEventRecord(ix->ev[1], ix->stream), "hipEventRecord");
     if (nq > 0) {
         song::SearchArgs a = make_args(ix, p, d_queries, nq, k_max, d_ids, d_dists, (int32_t*)d_trace);
         a.ks = d_ks;
         a.efs = d_efs;
+        HIP_TRY(run_plan(ix, p, a, ix->d_plan, ix->stream), "planning pass");
         HIP_TRY(song::launch_search(a, ix->metric, p.S, p.G, p.qch, p.shape, ix->stream), "search kernel launch");
     }
     HIP_TRY(hipEventRecord(ix->ev[2], ix->stream), "hipEventRecord");
@@ -598,9 +653,9 @@ int song_index_set_model(song_index* ix, const song_learned_model* m) {
     using song::LM_C;
     using song::LM_H;
     using song::LM_P;
-    // one allocation: mu | proj | scale | e_w1 | e_b1 | e_w2 | e_b2 | e_w3 | e_b3 | pivots | stop
+    // one allocation: mu | proj | scale | e_w1 | e_b1 | e_w2 | e_b2 | e_w3 | e_b3 | pivots | stop | cost
     const size_t sizes[] = {(size_t)pd, (size_t)LM_P * pd, LM_P, LM_H * LM_P, LM_H, LM_H * LM_H, LM_H,
-                            (size_t)LM_C * LM_H, LM_C, LM_C, song::LM_STOP_FLOATS};
+                            (size_t)LM_C * LM_H, LM_C, LM_C, song::LM_STOP_FLOATS, song::LM_COST_FLOATS};
     std::vector<size_t> offs;
     size_t total = 0;
     for (size_t sz : sizes) {
@@ -622,27 +677,44 @@ int song_index_set_model(song_index* ix, const song_learned_model* m) {
     put(8, m->entry_b3);
     std::memcpy(host.data() + offs[9], m->pivots, LM_C * 4);
     put(10, m->stop_weights);
+    put(11, m->cost_weights);
     std::lock_guard<std::mutex> lock(ix->mu);
     HIP_TRY(hipSetDevice(ix->device), "hipSetDevice");
-    if (!ix->d_model) HIP_TRY(hipMalloc(&ix->d_model, total * 4), "hipMalloc model");
+    if (ix->d_model && ix->model_floats < total) {
+        HIP_TRY(hipStreamSynchronize(ix->stream), "hipStreamSynchronize");
+        (void)hipFree(ix->d_model);
+        ix->d_model = nullptr;
+        ix->model_floats = 0;
+        ix->has_model = ix->has_cost = false;
+    }
+    if (!ix->d_model) {
+        HIP_TRY(hipMalloc(&ix->d_model, total * 4), "hipMalloc model");
+        ix->model_floats = total;
+    }
     HIP_TRY(hipMemcpy(ix->d_model, host.data(), total * 4, hipMemcpyHostToDevice), "model upload");
+    float* b = ix->d_model;
+    song::PlanArgs& P = ix->plan;
+    P = song::PlanArgs{};
+    P.dstride = ix->dstride;
+    P.n = ix->n;
+    P.mu = b + offs[0];
+    P.proj = b + offs[1];
+    P.proj_scale = b + offs[2];
+    P.e_w1 = b + offs[3];
+    P.e_b1 = b + offs[4];
+    P.e_w2 = b + offs[5];
+    P.e_b2 = b + offs[6];
+    P.e_w3 = b + offs[7];
+    P.e_b3 = b + offs[8];
+    P.pivots = (const int32_t*)(b + offs[9]);
+    P.c_weights = m->cost_weights ? b + offs[11] : nullptr;
     song::LearnedArgs& L = ix->model;
     L = song::LearnedArgs{};
-    float* b = ix->d_model;
-    L.mu = b + offs[0];
-    L.proj = b + offs[1];
-    L.proj_scale = b + offs[2];
-    L.e_w1 = b + offs[3];
-    L.e_b1 = b + offs[4];
-    L.e_w2 = b + offs[5];
-    L.e_b2 = b + offs[6];
-    L.e_w3 = b + offs[7];
-    L.e_b3 = b + offs[8];
-    L.pivots = (const int32_t*)(b + offs[9]);
     L.s_weights = b + offs[10];
     L.s_b3 = m->stop_b3;
     L.dist_scale = m->dist_scale;
     ix->has_model = true;
+    ix->has_cost = m->cost_weights != nullptr;
     return SONG_OK;
 }
 
@@ -661,6 +733,7 @@ int song_learn_trace(song_index* ix, const void* d_queries, int32_t nq, int32_t 
     if (rc) return rc;
     std::lock_guard<std::mutex> lock(ix->mu);
     HIP_TRY(hipSetDevice(ix->device), "hipSetDevice");
+    if (needs_plan(p) && (rc = ensure_plan_ws(ix, std::max<int32_t>(nq, 1), p.n_seeds))) return rc;
     int32_t* d_ids = nullptr;
     HIP_TRY(hipMalloc(&d_ids, (size_t)std::max(nq, 1) * k * 4), "hipMalloc ids");
     hipError_t e = hipSuccess;
@@ -670,7 +743,8 @@ int song_learn_trace(song_index* ix, const void* d_queries, int32_t nq, int32_t 
         a.L.trace_count = d_count;
         a.L.final_dk = d_final_dk;
         a.L.max_evals = max_evals;
-        e = song::launch_search(a, ix->metric, p.S, p.G, p.qch, p.shape, ix->stream);
+        e = run_plan(ix, p, a, ix->d_plan, ix->stream);
+        if (e == hipSuccess) e = song::launch_search(a, ix->metric, p.S, p.G, p.qch, p.shape, ix->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(ix->stream);
     }
     (void)hipFree(d_ids);
@@ -679,11 +753,14 @@ int song_learn_trace(song_index* ix, const void* d_queries, int32_t nq, int32_t 
 }
 
 int song_mlp_train(int32_t device, const float* X, const float* targets, int64_t n, int32_t in_dim, int32_t out_dim,
-                   int32_t epochs, int32_t batch, float lr, uint32_t seed, float* weights, float* loss_out) {
+                   int32_t loss, int32_t epochs, int32_t batch, float lr, uint32_t seed, float* weights,
+                   float* loss_out) {
     if (!X || !targets || !weights || n < 1 || epochs < 1 || batch < 1 || !(lr > 0.f))
         return fail(SONG_ERR_ARG, "bad training arguments");
-    if (!((in_dim == 8 && out_dim == 1) || (in_dim == 16 && out_dim == 64)))
-        return fail(SONG_ERR_ARG, "supported shapes: 8 -> 1 (stop), 16 -> 64 (entry)");
+    if (!((in_dim == 8 && out_dim == 1) || (in_dim == 16 && out_dim == 1) || (in_dim == 16 && out_dim == 64)))
+        return fail(SONG_ERR_ARG, "supported shapes: 8 -> 1 (stop), 16 -> 1 (cost), 16 -> 64 (entry)");
+    if (loss != SONG_LOSS_DEFAULT && loss != SONG_LOSS_MSE) return fail(SONG_ERR_ARG, "unknown loss");
+    if (loss == SONG_LOSS_MSE && out_dim != 1) return fail(SONG_ERR_ARG, "the squared-error loss needs out_dim == 1");
     if (out_dim > 1)
         for (int64_t i = 0; i < n; i++)
             if (!(targets[i] >= 0.f && targets[i] < (float)out_dim)) return fail(SONG_ERR_ARG, "class id out of range");
@@ -733,7 +810,8 @@ int song_mlp_train(int32_t device, const float* X, const float* targets, int64_t
             const int nw = (nb + per_wave - 1) / per_wave;
             step++;
             const float bc1 = 1.f - std::pow(0.9f, (float)step), bc2 = 1.f - std::pow(0.999f, (float)step);
-            e = song::launch_mlp_grad(in_dim, out_dim, dX, dT, dO, b0, nb, per_wave, dW, dP, nw, 0);
+            e = song::launch_mlp_grad(in_dim, out_dim, loss == SONG_LOSS_MSE, dX, dT, dO, b0, nb, per_wave, dW, dP,
+                                      nw, 0);
             if (e == hipSuccess) e = song::launch_mlp_adam(dW, dM, dV, dP, nw, total, nb, lr, bc1, bc2, dL + ep, 0);
         }
     }
@@ -751,8 +829,8 @@ int song_mlp_train(int32_t device, const float* X, const float* targets, int64_t
 int song_mlp_forward(int32_t device, const float* X, int64_t n, int32_t in_dim, int32_t out_dim,
                      const float* weights, float* logits_out) {
     if (!X || !weights || !logits_out || n < 0) return fail(SONG_ERR_ARG, "bad arguments");
-    if (!((in_dim == 8 && out_dim == 1) || (in_dim == 16 && out_dim == 64)))
-        return fail(SONG_ERR_ARG, "supported shapes: 8 -> 1 (stop), 16 -> 64 (entry)");
+    if (!((in_dim == 8 && out_dim == 1) || (in_dim == 16 && out_dim == 1) || (in_dim == 16 && out_dim == 64)))
+        return fail(SONG_ERR_ARG, "supported shapes: 8 -> 1 (stop), 16 -> 1 (cost), 16 -> 64 (entry)");
     if (n == 0) return SONG_OK;
     HIP_TRY(hipSetDevice(device), "hipSetDevice");
     const int total = song::mlp_param_count(in_dim, out_dim);
@@ -763,19 +841,25 @@ int song_mlp_forward(int32_t device, const float* X, int64_t n, int32_t in_dim, 
     if (e == hipSuccess) e = hipMemcpy(dX, X, (size_t)n * in_dim * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(dW, weights, (size_t)total * 4, hipMemcpyHostToDevice);
     song::LearnedArgs L{};
-    if (out_dim == 1) {
+    song::PlanArgs P{};
+    int which = 0;
+    if (in_dim == 8) {
         L.s_weights = dW;
         L.s_b3 = weights[song::LM_STOP_FLOATS];
+    } else if (out_dim == 1) {
+        which = 2;
+        P.c_weights = dW;
     } else {
         const int H = song::LM_H;
-        L.e_w1 = dW;
-        L.e_b1 = dW + H * 16;
-        L.e_w2 = L.e_b1 + H;
-        L.e_b2 = L.e_w2 + H * H;
-        L.e_w3 = L.e_b2 + H;
-        L.e_b3 = L.e_w3 + 64 * H;
+        which = 1;
+        P.e_w1 = dW;
+        P.e_b1 = dW + H * 16;
+        P.e_w2 = P.e_b1 + H;
+        P.e_b2 = P.e_w2 + H * H;
+        P.e_w3 = P.e_b2 + H;
+        P.e_b3 = P.e_w3 + 64 * H;
     }
-    if (e == hipSuccess) e = song::launch_learned_forward(L, out_dim == 1 ? 0 : 1, dX, n, dY, 0);
+    if (e == hipSuccess) e = song::launch_learned_forward(L, P, which, dX, n, dY, 0);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e == hipSuccess) e = hipMemcpy(logits_out, dY, (size_t)n * out_dim * 4, hipMemcpyDeviceToHost);
     for (void* q : {(void*)dX, (void*)dW, (void*)dY})

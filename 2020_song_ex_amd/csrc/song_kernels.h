@@ -15,8 +15,15 @@ constexpr int LM_F = 8;   // per-hop features (stop MLP input)
 constexpr int LM_H = 64;  // hidden width (one lane per unit)
 constexpr int LM_STOP_FLOATS = LM_H * LM_F + LM_H + LM_H * LM_H + LM_H + LM_H;  // w1 b1 w2 b2 w3
 
-struct LearnedArgs {
-    // entry: z = scale * (proj (q - mu)); scores = W3 relu(W2 relu(W1 z + b1) + b2) + b3
+constexpr int LM_COST_FLOATS = LM_H * LM_P + LM_H + LM_H * LM_H + LM_H + LM_H + 1;  // ... w3 b3
+
+// The query-side networks (entry, cost), run by the planning kernel before
+// the search: z = scale * (proj (q - mu)) feeds both.
+struct PlanArgs {
+    const void* queries;      // as SearchArgs::queries
+    int32_t nq;
+    int32_t dstride;
+    int64_t n;
     const float* mu;          // [dstride] (float metrics; unused for Hamming)
     const float* proj;        // [LM_P x proj_dim] row-major, proj_dim = dstride or 256 (Hamming, +-1 bits)
     const float* proj_scale;  // [LM_P]
@@ -27,12 +34,19 @@ struct LearnedArgs {
     const float* e_w3;        // [LM_C x LM_H]
     const float* e_b3;        // [LM_C]
     const int32_t* pivots;    // [LM_C] node ids (< n, checked on the host)
-    // stop: logit = w3 . relu(W2 relu(W1 f + b1) + b2) + b3, features f (LM_F)
+    const float* c_weights;   // LM_COST_FLOATS (16 -> 64 -> 64 -> 1), or null
+    int32_t n_entry;          // seeds per query
+    int32_t* entries;         // [nq x n_entry] out, or null (no entry network)
+    float* cost;              // [nq] out (predicted standardised log2 hops), or null
+};
+
+// The stop network, read by the search kernel.
+struct LearnedArgs {
+    // logit = w3 . relu(W2 relu(W1 f + b1) + b2) + b3, features f (LM_F)
     const float* s_weights;   // LM_STOP_FLOATS: w1 [H x F], b1 [H], w2 [H x H], b2 [H], w3 [H]
     float s_b3;
     float stop_logit;         // stop when logit > stop_logit
     float dist_scale;         // feature normalisation of distance differences
-    int32_t use_entry;
     int32_t use_stop;
     int32_t min_hops;         // first hop at which the stop MLP is consulted
     int32_t interval;         // consult every `interval` hops
@@ -63,7 +77,8 @@ struct SearchArgs {
     const int32_t* efs;      // per-query queue size, or null (uniform ef); ef := max(ef, k)
     int32_t k_stride;        // output row length (>= every k)
     int32_t* queue_head;     // persistent mode: device counter (zeroed at launch), else null
-    int32_t learned;         // 1: use L (entry / stop / trace as its flags say)
+    const int32_t* order;    // [nq] queue position -> query index, or null (identity)
+    int32_t learned;         // 1: use L (stop / trace as its flags say)
     LearnedArgs L;
 };
 
@@ -72,12 +87,20 @@ struct LaunchShape {
     int persistent_blocks;   // grid size in persistent mode
 };
 
-// metric: METRIC_*; S: set slots per lane (cap <= 64*S, S in {1,2,4,8});
-// G: lanes per row (power of two); qch: 16-byte chunks per lane (G*qch*4 >= dstride).
-// which = 0: stop network on X [n x 8] -> Y [n]; 1: entry network on X [n x 16] -> Y [n x 64].
-hipError_t launch_learned_forward(const LearnedArgs& L, int which, const float* X, int64_t n, float* Y,
-                                  hipStream_t st);
+// Forward passes of the kernels' own inference code on given inputs:
+// which = 0: stop network (L.s_weights) on X [n x 8] -> Y [n];
+// which = 1: entry network (P.e_*) on X [n x 16] -> Y [n x 64];
+// which = 2: cost network (P.c_weights) on X [n x 16] -> Y [n].
+hipError_t launch_learned_forward(const LearnedArgs& L, const PlanArgs& P, int which, const float* X, int64_t n,
+                                  float* Y, hipStream_t st);
 
+// Planning pass of the learned search: entry seeds and predicted cost per
+// query (one wave per query), then, when order is non-null, the queue order
+// by predicted cost, most expensive first (counting sort over 256 bins).
+hipError_t launch_plan(const PlanArgs& P, int metric, int32_t* order, hipStream_t st);
+
+// metric: METRIC_*; S: set slots per lane (cap <= 64*S, S in {1,2,3,4,6,8});
+// G: lanes per row (power of two); qch: 16-byte chunks per lane (G*qch*4 >= dstride).
 hipError_t launch_search(const SearchArgs& a, int metric, int S, int G, int qch, const LaunchShape& shape,
                          hipStream_t st);
 hipError_t launch_search_l2(const SearchArgs& a, int S, int G, int qch, const LaunchShape& shape, hipStream_t st);

@@ -407,24 +407,6 @@ __device__ __forceinline__ void hamming_distances(const ulonglong2* __restrict__
 
 // ---------------------------------------------------------------- learned entry / termination
 
-// One dense 64 -> 64 layer, one output unit per lane: lane j returns
-// b[j] + sum_i W[j][i] x_i, where x_i is held by lane i.  W row-major in global
-// memory (read-only, shared by every wave, so it stays in cache).
-__device__ __forceinline__ float dense64_global(const float* __restrict__ W, const float* __restrict__ b, float x,
-                                                int lane) {
-    const float4* row = (const float4*)(W + lane * 64);
-    float acc = b[lane];
-#pragma unroll
-    for (int i4 = 0; i4 < 16; i4++) {
-        const float4 w = row[i4];
-        acc = __builtin_fmaf(w.x, readlane_f(x, 4 * i4 + 0), acc);
-        acc = __builtin_fmaf(w.y, readlane_f(x, 4 * i4 + 1), acc);
-        acc = __builtin_fmaf(w.z, readlane_f(x, 4 * i4 + 2), acc);
-        acc = __builtin_fmaf(w.w, readlane_f(x, 4 * i4 + 3), acc);
-    }
-    return acc;
-}
-
 // The stop network's weights, staged once per workgroup in LDS: w1 [64][8],
 // b1 [64], w2 interleaved so lane j reads 16 conflict-free float4
 // (element (j, i) at (i/4)*256 + j*4 + i%4), b2 [64], w3 [64].
@@ -470,60 +452,6 @@ __device__ __forceinline__ float stop_logit(const float* lds, const float (&f)[L
     }
     acc = fmaxf(acc, 0.f);
     return wave_sum(acc * lds[SW_W3 + lane]) + b3;
-}
-
-// Entry network on wave-uniform inputs z: lane c returns the score of pivot c.
-__device__ __forceinline__ float entry_scores(const LearnedArgs& L, const float (&z)[LM_P], int lane) {
-    const float4* w1 = (const float4*)(L.e_w1 + lane * LM_P);
-    float h = L.e_b1[lane];
-#pragma unroll
-    for (int p4 = 0; p4 < LM_P / 4; p4++) {
-        const float4 w = w1[p4];
-        h = __builtin_fmaf(w.x, z[4 * p4 + 0], h);
-        h = __builtin_fmaf(w.y, z[4 * p4 + 1], h);
-        h = __builtin_fmaf(w.z, z[4 * p4 + 2], h);
-        h = __builtin_fmaf(w.w, z[4 * p4 + 3], h);
-    }
-    h = fmaxf(h, 0.f);
-    const float h2 = fmaxf(dense64_global(L.e_w2, L.e_b2, h, lane), 0.f);
-    return dense64_global(L.e_w3, L.e_b3, h2, lane);
-}
-
-// Entry vertices from the entry network: the n_entry highest-scoring pivots,
-// best first, in lanes 0..n_entry-1.  Returns false (the caller then keeps the
-// reference seeds) when any score is not a number.
-template <int METRIC>
-__device__ __forceinline__ bool learned_entry(const SearchArgs& P, int qid, int lane, int& sid) {
-    const LearnedArgs& L = P.L;
-    float z[LM_P];
-#pragma unroll
-    for (int p = 0; p < LM_P; p++) {
-        float part = 0.f;
-        if constexpr (METRIC == METRIC_HAMMING) {
-            // the code as a +-1 vector of 256 components; lane l owns bits 4l..4l+3
-            const uint64_t w = ((const uint64_t*)P.queries)[(size_t)qid * 4 + (lane >> 4)];
-            const int b0 = (lane & 15) * 4;
-            const float* pr = L.proj + p * 256 + lane * 4;
-#pragma unroll
-            for (int b = 0; b < 4; b++) part = __builtin_fmaf(pr[b], ((w >> (b0 + b)) & 1ull) ? 1.f : -1.f, part);
-        } else {
-            const float* q = (const float*)P.queries + (size_t)qid * P.dstride;
-            const float* pr = L.proj + (size_t)p * P.dstride;
-            for (int e = lane; e < P.dstride; e += 64) part = __builtin_fmaf(pr[e], q[e] - L.mu[e], part);
-        }
-        z[p] = wave_sum(part) * L.proj_scale[p];
-    }
-    float score = entry_scores(L, z, lane);
-    if (__ballot(!(score == score))) return false;
-    const int piv = L.pivots[lane];
-    for (int r = 0; r < P.n_entry; r++) {
-        const float m = wave_max(score);
-        const int l = ffs64(__ballot(score == m));
-        const int id = __builtin_amdgcn_readlane(piv, l);
-        if (lane == r) sid = id;
-        if (lane == l) score = -INFINITY;
-    }
-    return true;
 }
 
 // The k smallest distances seen so far, ascending in lanes 0..k-1 (k <= 64):
@@ -600,8 +528,12 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
     cand.init(ef);
     res.init(ef);
     // QueryTrace, telemetry.h:5-18 (wave-uniform counters)
-    int tr_seed = 0, tr_visited = 0, tr_expanded = 0, tr_slots = 0, tr_cand = 0, tr_devals = 0;
-    int tr_fpush = 0, tr_rupd = 0, tr_peakf = 0, tr_peakr = 0, tr_early = 0;
+    // (distance_evals = visited, candidate_nodes = visited - seeds,
+    // neighbor_slots = block per expansion that read its row, and the result
+    // set never shrinks before the trim, so its peak is its final size: those
+    // four are derived at the end instead of carried through the loop.)
+    int tr_seed = 0, tr_visited = 0, tr_expanded = 0;
+    int tr_fpush = 0, tr_rupd = 0, tr_peakf = 0, tr_early = 0;
     // learned termination state (k <= 64 on this path)
     const bool tracing = LEARN && P.L.trace != nullptr;
     const bool stopping = LEARN && P.L.use_stop && !tracing;
@@ -632,7 +564,6 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
         tr_fpush += cand.push_items(it_d, it_id, F, lane);
         tr_rupd += res.push_items(it_d, it_id, F, lane);
         tr_peakf = max(tr_peakf, cand.sz);
-        tr_peakr = max(tr_peakr, res.sz);
         if constexpr (LEARN) {
             const int before = tk.inserts;
             tk.insert(it_d, F, lane);
@@ -643,15 +574,11 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
     };
 
     // Seeds: kernalGraph.h:215-231 -- ids (size_t)p * n / n_seeds, or the
-    // caller's entry list.
+    // caller's (or the planning pass's) entry list.
     {
         int sid = 0;
         bool sv = lane < P.n_entry;
-        bool learned_seeds = false;
-        if constexpr (LEARN) {
-            if (P.L.use_entry) learned_seeds = learned_entry<METRIC>(P, qid, lane, sid);
-        }
-        if (sv && !learned_seeds) {
+        if (sv) {
             if (P.entries) {
                 sid = P.entries[(size_t)qid * P.n_entry + lane];
                 sv = sid >= 0 && sid < P.n;
@@ -662,7 +589,6 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
         const int F = visit_row(sid, sv);
         tr_seed += F;
         tr_visited += F;
-        tr_devals += F;
     }
 
     // Main loop: kernalGraph.h:233-289.
@@ -726,33 +652,75 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
                 }
             }
         }
-        tr_slots += P.block;
         const int nb = lane < P.block ? P.adj[(size_t)cn * P.block + lane] : -1;
-        const int F = visit_row(nb, nb >= 0);
-        tr_visited += F;
-        tr_cand += F;
-        tr_devals += F;
+        tr_visited += visit_row(nb, nb >= 0);
     }
+    const int tr_peakr = res.sz;
 
     // Trim to k, then drain best-first (kernalGraph.h:292-298, gpuHeap.h:90-108).
-    while (res.sz > k) {
-        const float M = res.vmax(lane);
-        res.remove_at(res.first_eq(M, lane), lane);
-    }
-    res.mx_ok = false;
+    // When each of the cnt smallest distances occurs once in the set, the
+    // trim removes exactly the entries above the cnt-th smallest and the drain
+    // returns the rest in ascending order, wherever they sit: extract them
+    // directly, one per lane.  Any repeated value among them falls back to the
+    // step-by-step replay, whose first-index rules decide ties.
     const int cnt = res.sz < k ? res.sz : k;
     int32_t* out = P.ids_out + (size_t)qid * P.k_stride;
     float* dout = P.dists_out ? P.dists_out + (size_t)qid * P.k_stride : nullptr;
-    for (int i = 0; i < cnt; i++) {
-        const float m = res.vmin(lane);
-        const int p = res.first_eq(m, lane);
-        float dv;
-        int iv;
-        res.get(p, dv, iv);
-        res.remove_at(p, lane);
-        if (lane == 0) {
-            out[i] = iv;
-            if (dout) dout[i] = dv;
+    bool fast = cnt <= 64;
+    if (fast) {
+        typename WSet<S>::fvec t = res.d;
+        float od = INFINITY;
+        int oi = -1;
+        for (int i = 0; i < cnt; i++) {
+            float m = INFINITY;
+#pragma unroll
+            for (int s = 0; s < S; s++) m = fminf(m, t[s]);
+            m = wave_min(m);
+            int c = 0, p = -1;
+#pragma unroll
+            for (int s = 0; s < S; s++) {
+                const uint64_t b = __ballot(t[s] == m);
+                c += __popcll(b);
+                if (p < 0 && b) p = s * 64 + ffs64(b);
+            }
+            if (c != 1) {
+                fast = false;
+                break;
+            }
+            float dv;
+            int iv;
+            res.get(p, dv, iv);
+            if (lane == i) {
+                od = dv;
+                oi = iv;
+            }
+            const int slot = p >> 6;
+            const bool mine = lane == (p & 63);
+#pragma unroll
+            for (int s = 0; s < S; s++) t[s] = (mine && slot == s) ? __builtin_nanf("") : t[s];
+        }
+        if (fast && lane < cnt) {
+            out[lane] = oi;
+            if (dout) dout[lane] = od;
+        }
+    }
+    if (!fast) {
+        while (res.sz > k) {
+            const float M = res.vmax(lane);
+            res.remove_at(res.first_eq(M, lane), lane);
+        }
+        res.mx_ok = false;
+        for (int i = 0; i < cnt; i++) {
+            const float m = res.vmin(lane);
+            const int p = res.first_eq(m, lane);
+            float dv;
+            int iv;
+            res.get(p, dv, iv);
+            res.remove_at(p, lane);
+            if (lane == 0) {
+                out[i] = iv;
+                if (dout) dout[i] = dv;
+            }
         }
     }
     for (int i = cnt + lane; i < P.k_stride; i += 64) {
@@ -768,9 +736,9 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
         t[0] = tr_seed;
         t[1] = tr_visited;
         t[2] = tr_expanded;
-        t[3] = tr_slots;
-        t[4] = tr_cand;
-        t[5] = tr_devals;
+        t[3] = (tr_expanded - (tr_early ? 1 : 0)) * P.block;
+        t[4] = tr_visited - tr_seed;
+        t[5] = tr_visited;
         t[6] = tr_fpush;
         t[7] = tr_rupd;
         t[8] = tr_peakf;
@@ -802,8 +770,9 @@ __global__ void __launch_bounds__(256) search_kernel(SearchArgs P) {
         }
     }
     const int wave = threadIdx.x >> 6;
-    const int qid = blockIdx.x * (blockDim.x >> 6) + wave;
-    if (qid >= P.nq) return;
+    const int pos = blockIdx.x * (blockDim.x >> 6) + wave;
+    if (pos >= P.nq) return;
+    const int qid = P.order ? P.order[pos] : pos;
     search_query<METRIC, S, G, QCH, LEARN>(P, qid, lane_id(), waves_lds + (size_t)wave * (P.bloom_words + 64),
                                            stop_lds);
 }
@@ -832,7 +801,7 @@ __global__ void __launch_bounds__(256) persistent_kernel(SearchArgs P) {
         if (lane == 0) item = atomicAdd(P.queue_head, 1);
         item = __builtin_amdgcn_readfirstlane(item);
         if (item >= P.nq) break;
-        search_query<METRIC, S, G, QCH, LEARN>(P, item, lane, lds, stop_lds);
+        search_query<METRIC, S, G, QCH, LEARN>(P, P.order ? P.order[item] : item, lane, lds, stop_lds);
     }
 }
 
@@ -895,7 +864,9 @@ hipError_t launch_s(const SearchArgs& a, int S, int G, int qch, const LaunchShap
     switch (S) {
     case 1: return launch_g<METRIC, 1>(a, G, qch, wpb, st);
     case 2: return launch_g<METRIC, 2>(a, G, qch, wpb, st);
+    case 3: return launch_g<METRIC, 3>(a, G, qch, wpb, st);
     case 4: return launch_g<METRIC, 4>(a, G, qch, wpb, st);
+    case 6: return launch_g<METRIC, 6>(a, G, qch, wpb, st);
     case 8: return launch_g<METRIC, 8>(a, G, qch, wpb, st);
     }
     return hipErrorInvalidValue;

@@ -11,7 +11,8 @@
 // options, never on what it was batched with.
 //
 // Memory budget: every admitted request holds row_bytes + 8 + 8*k_max bytes
-// of device state until its slot completes; admission keeps the sum at or
+// of device state (plus its planning workspace in the learned search) until
+// its slot completes; admission keeps the sum at or
 // under the budget (requests that do not fit wait, counted as deferred).
 #include <hip/hip_runtime.h>
 
@@ -60,6 +61,7 @@ struct Slot {
     int32_t* d_ids = nullptr;
     float* d_dists = nullptr;
     int32_t* d_counter = nullptr;
+    int32_t* d_plan = nullptr;  // planning workspace of the learned search
     std::vector<Request*> reqs;
     int64_t bytes = 0;
     bool busy = false;
@@ -116,7 +118,7 @@ hipError_t launch_slot(song_server* sv, song::Slot& s) {
     if (e == hipSuccess) e = hipMemcpyAsync(s.d_efs, s.h_efs, (size_t)nb * 4, hipMemcpyHostToDevice, s.stream);
     if (e != hipSuccess) return e;
     int rc = song::launch_mixed_on_stream(sv->ix, s.d_q, nb, s.d_ks, s.d_efs, kst, efmax, &sv->opts.search, s.d_ids,
-                                          s.d_dists, s.d_counter, s.stream);
+                                          s.d_dists, s.d_counter, s.d_plan, s.stream);
     if (rc != SONG_OK) return hipErrorInvalidValue;
     e = hipMemcpyAsync(s.h_ids, s.d_ids, (size_t)nb * kst * 4, hipMemcpyDeviceToHost, s.stream);
     if (e == hipSuccess) e = hipMemcpyAsync(s.h_dists, s.d_dists, (size_t)nb * kst * 4, hipMemcpyDeviceToHost, s.stream);
@@ -200,7 +202,8 @@ void free_slot(song::Slot& s) {
     if (s.stream) (void)hipStreamSynchronize(s.stream);
     for (void* p : {(void*)s.h_q, (void*)s.h_ks, (void*)s.h_efs, (void*)s.h_ids, (void*)s.h_dists})
         if (p) (void)hipHostFree(p);
-    for (void* p : {(void*)s.d_q, (void*)s.d_ks, (void*)s.d_efs, (void*)s.d_ids, (void*)s.d_dists, (void*)s.d_counter})
+    for (void* p : {(void*)s.d_q, (void*)s.d_ks, (void*)s.d_efs, (void*)s.d_ids, (void*)s.d_dists, (void*)s.d_counter,
+                    (void*)s.d_plan})
         if (p) (void)hipFree(p);
     if (s.done) (void)hipEventDestroy(s.done);
     if (s.stream) (void)hipStreamDestroy(s.stream);
@@ -233,7 +236,8 @@ int song_server_create(song_index* ix, const song_server_opts* o, song_server** 
     sv->opts = opts;
     sv->device = device;
     sv->row_bytes = (size_t)stride * (metric == SONG_METRIC_HAMMING ? 8 : 4);
-    sv->req_bytes = (int64_t)sv->row_bytes + 8 + 8 * (int64_t)opts.k_max;
+    sv->req_bytes = (int64_t)sv->row_bytes + 8 + 8 * (int64_t)opts.k_max +
+                    (opts.search.learned ? 4 * song::plan_ws_words(1) : 0);
     sv->capacity = (int)std::max<int64_t>(1, std::min<int64_t>(opts.max_batch, opts.memory_budget / sv->req_bytes));
     if (opts.memory_budget < sv->req_bytes) {
         delete sv;
@@ -259,6 +263,8 @@ int song_server_create(song_index* ix, const song_server_opts* o, song_server** 
         if (e == hipSuccess) e = hipMalloc((void**)&s.d_ids, cap * kst * 4);
         if (e == hipSuccess) e = hipMalloc((void**)&s.d_dists, cap * kst * 4);
         if (e == hipSuccess) e = hipMalloc((void**)&s.d_counter, 64);
+        if (e == hipSuccess && opts.search.learned)
+            e = hipMalloc((void**)&s.d_plan, (size_t)song::plan_ws_words((int64_t)cap) * 4);
         if (e != hipSuccess) {
             for (auto& t : sv->slots) free_slot(t);
             delete sv;

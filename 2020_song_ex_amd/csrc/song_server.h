@@ -12,9 +12,11 @@ namespace song {
 void set_last_error(const std::string& msg);
 void index_info(const song_index* ix, int* metric, int* device);
 // Per-request budgets, launched on `stream` with the caller's queue counter
-// (persistent mode); no synchronisation.
+// (persistent mode) and planning workspace (learned entry / cost order,
+// plan_ws_words(nq) int32); no synchronisation.
 int launch_mixed_on_stream(song_index* ix, const void* d_q, int nq, const int32_t* d_ks, const int32_t* d_efs,
                            int k_stride, int ef_max, const song_search_opts* opts, int32_t* d_ids, float* d_dists,
-                           int32_t* d_counter, hipStream_t stream);
+                           int32_t* d_counter, int32_t* d_plan_ws, hipStream_t stream);
+int64_t plan_ws_words(int64_t nq);
 
 }  // namespace song

@@ -98,7 +98,7 @@ __device__ __forceinline__ void forward(const Lds<IN, OUT>& s, const float (&x)[
 template <int IN, int OUT>
 __global__ void __launch_bounds__(64) mlp_grad_kernel(const float* __restrict__ X, const float* __restrict__ T,
                                                       const int32_t* __restrict__ order, int64_t b0, int nb,
-                                                      int per_wave, const float* __restrict__ W,
+                                                      int per_wave, int mse, const float* __restrict__ W,
                                                       float* __restrict__ partial) {
     __shared__ Lds<IN, OUT> s;
     stage<IN, OUT>(W, s);
@@ -123,9 +123,14 @@ __global__ void __launch_bounds__(64) mlp_grad_kernel(const float* __restrict__ 
         float g = 0.f;  // dLoss/d output (lane c)
         if constexpr (OUT == 1) {
             const float z = rl(o, 0), y = T[idx];
-            const float p = 1.f / (1.f + __expf(-z));
-            if (lane == 0) g = p - y;
-            loss += fmaxf(z, 0.f) - z * y + __logf(1.f + __expf(-fabsf(z)));
+            if (mse) {  // squared error / 2 on a real-valued target
+                if (lane == 0) g = z - y;
+                loss += 0.5f * (z - y) * (z - y);
+            } else {    // logistic loss on a 0/1 label
+                const float p = 1.f / (1.f + __expf(-z));
+                if (lane == 0) g = p - y;
+                loss += fmaxf(z, 0.f) - z * y + __logf(1.f + __expf(-fabsf(z)));
+            }
         } else {
             const int label = (int)T[idx];
             const float m = xmax(lane < OUT ? o : -INFINITY);
@@ -211,14 +216,17 @@ __global__ void __launch_bounds__(64) mlp_forward_train_layout(const float* __re
 
 int mlp_param_count(int in, int out) { return offsets(in, out).total; }
 
-hipError_t launch_mlp_grad(int in, int out, const float* X, const float* T, const int32_t* order, int64_t b0, int nb,
-                           int per_wave, const float* W, float* partial, int nwaves, hipStream_t st) {
+hipError_t launch_mlp_grad(int in, int out, int mse, const float* X, const float* T, const int32_t* order, int64_t b0,
+                           int nb, int per_wave, const float* W, float* partial, int nwaves, hipStream_t st) {
     if (in == 8 && out == 1)
-        hipLaunchKernelGGL((mlp_grad_kernel<8, 1>), dim3(nwaves), dim3(64), 0, st, X, T, order, b0, nb, per_wave, W,
-                           partial);
+        hipLaunchKernelGGL((mlp_grad_kernel<8, 1>), dim3(nwaves), dim3(64), 0, st, X, T, order, b0, nb, per_wave, mse,
+                           W, partial);
+    else if (in == 16 && out == 1)
+        hipLaunchKernelGGL((mlp_grad_kernel<16, 1>), dim3(nwaves), dim3(64), 0, st, X, T, order, b0, nb, per_wave, mse,
+                           W, partial);
     else if (in == 16 && out == 64)
-        hipLaunchKernelGGL((mlp_grad_kernel<16, 64>), dim3(nwaves), dim3(64), 0, st, X, T, order, b0, nb, per_wave, W,
-                           partial);
+        hipLaunchKernelGGL((mlp_grad_kernel<16, 64>), dim3(nwaves), dim3(64), 0, st, X, T, order, b0, nb, per_wave, 0,
+                           W, partial);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
@@ -236,6 +244,8 @@ hipError_t launch_mlp_forward_train_layout(int in, int out, const float* X, int6
     const int blocks = (int)(n < 4096 ? (n > 0 ? n : 1) : 4096);
     if (in == 8 && out == 1)
         hipLaunchKernelGGL((mlp_forward_train_layout<8, 1>), dim3(blocks), dim3(64), 0, st, X, n, W, Y);
+    else if (in == 16 && out == 1)
+        hipLaunchKernelGGL((mlp_forward_train_layout<16, 1>), dim3(blocks), dim3(64), 0, st, X, n, W, Y);
     else if (in == 16 && out == 64)
         hipLaunchKernelGGL((mlp_forward_train_layout<16, 64>), dim3(blocks), dim3(64), 0, st, X, n, W, Y);
     else

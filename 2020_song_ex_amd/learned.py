@@ -1,25 +1,29 @@
-"""Learned entry and early termination for the graph search.
+"""Learned entry, scheduling and early termination for the graph search.
 
-Two 2 x 64 ReLU MLPs (FP32), trained on the device by libsong's HIP trainer
-(song_mlp_train) and run fused inside the search kernel:
+Three 2 x 64 ReLU MLPs (FP32), trained on the device by libsong's HIP trainer
+(song_mlp_train).  The query's 16 leading principal components (of a base
+sample, standardised) feed the first two, which run in a planning pass before
+the search (song_plan.hip); the third runs fused inside the search kernel:
 
-* entry network: the query's 16 leading principal components (of a base
-  sample, standardised) -> scores over 64
-  pivot nodes (nodes nearest to k-means centroids of the base); the best
-  n_seeds pivots replace the reference's evenly spaced seeds
-  (kernalGraph.h:215-216).  Label: the pivot nearest to the query.
+* entry network: projection -> scores over 64 pivot nodes (nodes nearest to
+  k-means centroids of the base); the best n_seeds pivots replace the
+  reference's evenly spaced seeds (kernalGraph.h:215-216).  Label: the pivot
+  nearest to the query.
+* cost network: projection -> the query's standardised log2 hop count (squared
+  error).  The persistent queue runs the queries most expensive first, so the
+  long searches overlap the short ones instead of trailing after them.
 * stop network: 8 features of where the search stands -- hop count, popped
   distance minus the k-th best, spread of the top k, hops since the top k last
   changed, frontier entries still closer than the k-th best, recent top-k
   insertions, frontier fill, recent improvement of the k-th best (distances
   divided by a per-index scale; song_search_impl.h) -> probability that the
-  k-th best distance is already final.  Labels come from traces of the search itself (song_learn_trace):
-  at a consulted hop t the label is 1 when the k-th best distance at t equals
-  the one at the end of the search.
+  k-th best distance is already final.  Labels come from traces of the search
+  itself (song_learn_trace): at a consulted hop t the label is 1 when the k-th
+  best distance at t equals the one at the end of the search.
 
 The plain kernel stays the fallback: a NaN entry score keeps the reference
-seeds and a NaN stop logit never stops, so a broken model returns the plain
-search's result.
+seeds, a NaN cost sorts as zero and a NaN stop logit never stops, so a broken
+model returns the plain search's result.
 """
 from __future__ import annotations
 
@@ -32,6 +36,7 @@ from .index import Index
 
 P_DIM, N_PIVOTS, N_FEATURES, HIDDEN = 16, 64, 8, 64
 STOP_FLOATS = HIDDEN * N_FEATURES + HIDDEN + HIDDEN * HIDDEN + HIDDEN + HIDDEN  # + b3
+LOSS_DEFAULT, LOSS_MSE = 0, 1
 
 
 class SongLearnedModel(ctypes.Structure):
@@ -39,7 +44,7 @@ class SongLearnedModel(ctypes.Structure):
                 ("proj_scale", ctypes.c_void_p), ("entry_w1", ctypes.c_void_p), ("entry_b1", ctypes.c_void_p),
                 ("entry_w2", ctypes.c_void_p), ("entry_b2", ctypes.c_void_p), ("entry_w3", ctypes.c_void_p),
                 ("entry_b3", ctypes.c_void_p), ("pivots", ctypes.c_void_p), ("stop_weights", ctypes.c_void_p),
-                ("stop_b3", ctypes.c_float), ("dist_scale", ctypes.c_float)]
+                ("stop_b3", ctypes.c_float), ("dist_scale", ctypes.c_float), ("cost_weights", ctypes.c_void_p)]
 
 
 def _bind():
@@ -51,7 +56,8 @@ def _bind():
                                        ctypes.c_void_p, ctypes.c_void_p]
         L.song_mlp_train.argtypes = [ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                      ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
-                                     ctypes.c_float, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p]
+                                     ctypes.c_int32, ctypes.c_float, ctypes.c_uint32, ctypes.c_void_p,
+                                     ctypes.c_void_p]
         L.song_mlp_forward.argtypes = [ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32,
                                        ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p]
         L._learned_bound = True
@@ -88,16 +94,17 @@ def mlp_forward_numpy(w, X, in_dim, out_dim):
     return h @ w3.T + b3
 
 
-def mlp_train(X, y, in_dim, out_dim, epochs=10, batch=1024, lr=2e-3, seed=0, weights=None, device=0):
+def mlp_train(X, y, in_dim, out_dim, epochs=10, batch=1024, lr=2e-3, seed=0, weights=None, device=0,
+              loss=LOSS_DEFAULT):
     """song_mlp_train: Adam on the device; returns (weights, per-epoch loss)."""
     L = _bind()
     X = np.ascontiguousarray(X, dtype=np.float32)
     y = np.ascontiguousarray(y, dtype=np.float32)
     w = mlp_init(in_dim, out_dim, seed) if weights is None else np.array(weights, dtype=np.float32)
-    loss = np.zeros(epochs, np.float32)
-    N.check(L.song_mlp_train(device, N.ptr(X), N.ptr(y), X.shape[0], in_dim, out_dim, epochs, batch, lr,
-                             seed, N.ptr(w), N.ptr(loss)), "song_mlp_train")
-    return w, loss
+    curve = np.zeros(epochs, np.float32)
+    N.check(L.song_mlp_train(device, N.ptr(X), N.ptr(y), X.shape[0], in_dim, out_dim, loss, epochs, batch, lr,
+                             seed, N.ptr(w), N.ptr(curve)), "song_mlp_train")
+    return w, curve
 
 
 def mlp_forward(w, X, in_dim, out_dim, device=0):
@@ -116,7 +123,7 @@ def mlp_forward(w, X, in_dim, out_dim, device=0):
 class LearnedModel:
     """Arrays of a learned model; set_on() uploads them to an index."""
 
-    def __init__(self, proj_dim, mu, proj, proj_scale, entry_w, pivots, stop_w, dist_scale):
+    def __init__(self, proj_dim, mu, proj, proj_scale, entry_w, pivots, stop_w, dist_scale, cost_w=None):
         self.proj_dim = int(proj_dim)
         self.mu = np.ascontiguousarray(mu, np.float32)
         self.proj = np.ascontiguousarray(proj, np.float32)
@@ -125,6 +132,7 @@ class LearnedModel:
         self.pivots = np.ascontiguousarray(pivots, np.int32)
         self.stop_w = np.ascontiguousarray(stop_w, np.float32)     # flat 8 -> 1 layout (b3 last)
         self.dist_scale = float(dist_scale)
+        self.cost_w = None if cost_w is None else np.ascontiguousarray(cost_w, np.float32)  # flat 16 -> 1
 
     def set_on(self, index: Index):
         L = _bind()
@@ -141,17 +149,19 @@ class LearnedModel:
         m.stop_weights = self.stop_w.ctypes.data
         m.stop_b3 = float(self.stop_w[STOP_FLOATS])
         m.dist_scale = self.dist_scale
+        m.cost_weights = self.cost_w.ctypes.data if self.cost_w is not None else None
         N.check(L.song_index_set_model(index.handle, ctypes.byref(m)), "song_index_set_model")
 
     def save(self, path):
+        extra = {} if self.cost_w is None else {"cost_w": self.cost_w}
         np.savez(path, proj_dim=self.proj_dim, mu=self.mu, proj=self.proj, proj_scale=self.proj_scale,
-                 entry_w=self.entry_w, pivots=self.pivots, stop_w=self.stop_w, dist_scale=self.dist_scale)
+                 entry_w=self.entry_w, pivots=self.pivots, stop_w=self.stop_w, dist_scale=self.dist_scale, **extra)
 
     @classmethod
     def load(cls, path):
         z = np.load(path)
         return cls(int(z["proj_dim"]), z["mu"], z["proj"], z["proj_scale"], z["entry_w"], z["pivots"],
-                   z["stop_w"], float(z["dist_scale"]))
+                   z["stop_w"], float(z["dist_scale"]), z["cost_w"] if "cost_w" in z.files else None)
 
     def project(self, queries, metric):
         """The entry network's input, as the kernel computes it (float64 here)."""
@@ -207,8 +217,11 @@ def kmeans_pivots(index: Index, base, n_pivots=N_PIVOTS, sample=50_000, iters=12
 
 
 def fit(index: Index, base, train_queries, *, k=10, ef=256, n_seeds=4, bloom_bits=0, stop_min_hops=8,
-        stop_interval=2, max_evals=128, entry_epochs=20, stop_epochs=4, seed=0, log=None):
-    """Train both networks for this index; returns the LearnedModel (already set on the index)."""
+        stop_interval=2, max_evals=128, entry_epochs=20, stop_epochs=4, cost_epochs=20, cost_ef=None,
+        cost_threshold=2.5, seed=0, log=None):
+    """Train the three networks for this index; returns the LearnedModel (already set on the index).
+    The cost network learns the hop counts of the learned search (entry + stop at cost_threshold) at
+    queue size cost_ef (default ef)."""
     say = log or (lambda *_: None)
     metric = index.metric
     proj_dim = 256 if metric == N.METRIC_HAMMING else index.row_stride
@@ -281,5 +294,18 @@ def fit(index: Index, base, train_queries, *, k=10, ef=256, n_seeds=4, bloom_bit
     sw, sloss = mlp_train(feats, y, N_FEATURES, 1, epochs=stop_epochs, lr=2e-3, seed=seed + 1)
     model.stop_w = sw
     say(f"stop network: loss {sloss[0]:.3f} -> {sloss[-1]:.3f}")
+    model.set_on(index)
+
+    # cost labels: standardised log2 hop counts of the learned search itself
+    r = index.search(train_queries, k, cost_ef or ef, bloom_bits=bloom_bits, n_seeds=n_seeds, learned=3,
+                     stop_threshold=cost_threshold, stop_min_hops=stop_min_hops, stop_interval=stop_interval)
+    hops = np.log2(np.maximum(r.trace[:, 2], 1).astype(np.float64))
+    mean, std = hops.mean(), max(hops.std(), 1e-6)
+    target = ((hops - mean) / std).astype(np.float32)
+    cw, closs = mlp_train(z, target, P_DIM, 1, epochs=cost_epochs, lr=2e-3, seed=seed + 2, loss=LOSS_MSE)
+    pred = mlp_forward_numpy(cw, z, P_DIM, 1)[:, 0]
+    corr = float(np.corrcoef(pred, target)[0, 1]) if target.std() > 0 else 0.0
+    say(f"cost network: loss {closs[0]:.3f} -> {closs[-1]:.3f}, correlation with log2 hops {corr:.3f}")
+    model.cost_w = cw
     model.set_on(index)
     return model

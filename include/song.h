@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define SONG_ABI_VERSION 1
+#define SONG_ABI_VERSION 2
 
 enum {
     SONG_METRIC_L2 = 0,
@@ -107,7 +107,7 @@ typedef struct song_search_opts {
     int32_t lanes_per_row;   /* lanes that share one vector row (power of
                                 two, 2..64); 0 = chosen from dim.  Fixes the
                                 float summation order (xor butterfly over
-                                the group, see song_kernels.hip). */
+                                the group, see song_search_impl.h). */
     int32_t mode;            /* SONG_MODE_PLAIN: one wave per query;
                                 SONG_MODE_PERSISTENT: a fixed grid drains a
                                 device query queue.  Same results. */
@@ -115,8 +115,13 @@ typedef struct song_search_opts {
                                 1..32; 0 = 32 */
     int32_t learned;         /* bit 0: entry vertices from the index's entry
                                 network; bit 1: stop when the stop network's
-                                logit exceeds stop_threshold.  Needs a model
-                                (song_index_set_model), k <= 64, dim <= 1024. */
+                                logit exceeds stop_threshold; bit 2: run the
+                                queries in order of the cost network's
+                                prediction, most expensive first (results are
+                                unchanged, only when each query starts).
+                                Needs a model (song_index_set_model; bit 2 a
+                                model with a cost network), k <= 64,
+                                dim <= 1024. */
     float stop_threshold;    /* logit threshold of the stop network */
     int32_t stop_min_hops;   /* first hop at which it is consulted */
     int32_t stop_interval;   /* consult every this many hops (>= 1) */
@@ -234,13 +239,18 @@ int song_search_device_mixed(song_index* index, const void* d_queries, int32_t n
 /* ---------------------------------------------------------------- learned entry / termination */
 
 /* A trained model for the learned search (no counterpart in the reference).
- * Two ReLU MLPs with 2 x 64 hidden units, FP32, row-major weights:
- *   entry: z = proj_scale * (proj (q - mu)), 16 inputs -> 64 -> 64 -> 64
- *          scores over the 64 pivot nodes; the n_seeds best are the seeds.
- *          For HAMMING q is the code as a +-1 vector of 256 and mu is unused.
+ * ReLU MLPs with 2 x 64 hidden units, FP32, row-major weights; the query
+ * projection z = proj_scale * (proj (q - mu)) (16 values) feeds the first two:
+ *   entry: z -> 64 -> 64 -> 64 scores over the 64 pivot nodes; the n_seeds
+ *          best are the seeds.  For HAMMING q is the code as a +-1 vector of
+ *          256 and mu is unused.
+ *   cost:  z -> 64 -> 64 -> 1, the query's predicted standardised log2 hop
+ *          count (cost_weights = w1 [64x16], b1 [64], w2 [64x64], b2 [64],
+ *          w3 [64], b3 [1]); optional (NULL: no cost order).
  *   stop:  8 per-hop features -> 64 -> 64 -> 1 logit
  *          (stop_weights = w1 [64x8], b1 [64], w2 [64x64], b2 [64], w3 [64]).
- * The features are defined in song_kernels.hip (search_query). */
+ * Entry and cost run in a planning pass before the search (song_plan.hip);
+ * the stop features are defined in the search kernel (song_search_impl.h). */
 typedef struct song_learned_model {
     int32_t proj_dim;        /* row stride of the index (dim rounded up to 4), or 256 */
     const float* mu;         /* [proj_dim] or NULL (HAMMING) */
@@ -256,6 +266,7 @@ typedef struct song_learned_model {
     const float* stop_weights; /* [4800] */
     float stop_b3;
     float dist_scale;        /* > 0: distance differences are divided by it */
+    const float* cost_weights; /* [5313] or NULL */
 } song_learned_model;
 
 /* Copies a model to the index's device (replacing any previous one). */
@@ -270,15 +281,20 @@ int song_learn_trace(song_index* index, const void* d_queries, int32_t nq, int32
                      const song_search_opts* opts, int32_t max_evals, float* d_trace, int32_t* d_count,
                      float* d_final_dk);
 
+enum {
+    SONG_LOSS_DEFAULT = 0, /* logistic on 0/1 labels (out_dim 1), softmax on class ids (out_dim > 1) */
+    SONG_LOSS_MSE = 1      /* squared error on real-valued targets (out_dim 1) */
+};
+
 /* Trains a 2 x 64 ReLU MLP (in_dim -> 64 -> 64 -> out_dim) on the device with
- * Adam.  X: host float [n x in_dim]; targets: host float [n] -- 0/1 labels
- * (out_dim == 1, logistic loss) or class ids (out_dim > 1, softmax loss).
+ * Adam.  Shapes: 8 -> 1 (stop), 16 -> 1 (cost), 16 -> 64 (entry).
+ * X: host float [n x in_dim]; targets: host float [n] as `loss` says.
  * weights: host float, w1 [64 x in] b1 [64] w2 [64x64] b2 [64] w3 [out x 64]
  * b3 [out]; read as the starting point and overwritten with the result.
  * loss_out (nullable): [epochs] mean training loss per epoch.
  * Deterministic for a given seed (sample order) and input. */
 int song_mlp_train(int32_t device, const float* X, const float* targets, int64_t n, int32_t in_dim,
-                   int32_t out_dim, int32_t epochs, int32_t batch, float lr, uint32_t seed,
+                   int32_t out_dim, int32_t loss, int32_t epochs, int32_t batch, float lr, uint32_t seed,
                    float* weights, float* loss_out);
 
 /* The same network's forward pass on the device (the inference code the

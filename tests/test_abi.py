@@ -26,7 +26,7 @@ def test_header_symbols_exported(song):
 
 
 def test_abi_version(song):
-    assert song._native.lib().song_abi_version() == 1
+    assert song._native.lib().song_abi_version() == 2
 
 
 def test_oracle_libraries_load(oracle):

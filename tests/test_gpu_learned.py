@@ -10,7 +10,7 @@ pytestmark = pytest.mark.gpu
 learned = importlib.import_module("2020_song_ex_amd.learned")
 
 
-@pytest.mark.parametrize("in_dim,out_dim", [(8, 1), (16, 64)])
+@pytest.mark.parametrize("in_dim,out_dim", [(8, 1), (16, 1), (16, 64)])
 def test_kernel_inference_matches_fp32_mlp(in_dim, out_dim):
     """The search kernel's fused MLP code against a float64-accumulated FP32 MLP (1e-5 relative)."""
     rng = np.random.default_rng(in_dim)
@@ -36,6 +36,17 @@ def test_training_fits_a_simple_rule():
     np.testing.assert_array_equal(w, w2)
 
 
+def test_training_regression():
+    """Squared-error training (the cost network's loss) fits a smooth target."""
+    rng = np.random.default_rng(4)
+    X = rng.normal(size=(20000, 16)).astype(np.float32)
+    y = (0.8 * X[:, 0] - 0.5 * X[:, 5] + 0.3 * np.abs(X[:, 2])).astype(np.float32)
+    w, loss = learned.mlp_train(X, y, 16, 1, epochs=8, lr=3e-3, seed=3, loss=learned.LOSS_MSE)
+    assert loss[-1] < loss[0] * 0.2, loss
+    pred = learned.mlp_forward(w, X, 16, 1)[:, 0]
+    assert np.corrcoef(pred, y)[0, 1] > 0.98
+
+
 def test_training_softmax():
     rng = np.random.default_rng(1)
     centers = rng.normal(size=(64, 16)).astype(np.float32) * 3
@@ -53,7 +64,8 @@ def setup(song, datasets):
     adj = song.nsw_build(base, block=32, exact_order=False)
     idx = song.Index(base, adj)
     train = datasets.gmm_int(20000, 3000, dim=64, n_centres=40, sigma=20.0, seed=5)[1][200:]
-    model = learned.fit(idx, base, train, k=10, ef=128, entry_epochs=10, stop_epochs=3)
+    model = learned.fit(idx, base, train, k=10, ef=128, entry_epochs=10, stop_epochs=3, cost_epochs=10,
+                        stop_min_hops=8, stop_interval=2)
     yield base, q, adj, idx, model
     idx.close()
 
@@ -95,6 +107,55 @@ def test_learned_entry_matches_oracle_with_those_seeds(song, oracle, setup):
     # the projection is computed in a different summation order than numpy's, so
     # a near-tie between two pivot scores can pick another seed list
     assert agree > 0.97, agree
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_cost_order_keeps_results(song, setup, mode):
+    """Running the queue in predicted-cost order changes when a query runs, not its result."""
+    base, q, adj, idx, model = setup
+    assert model.cost_w is not None
+    for base_flags in (1, 3):
+        a = idx.search(q, 10, 128, learned=base_flags, stop_threshold=2.0, stop_min_hops=8, stop_interval=2,
+                       mode=mode)
+        b = idx.search(q, 10, 128, learned=base_flags | 4, stop_threshold=2.0, stop_min_hops=8,
+                       stop_interval=2, mode=mode)
+        np.testing.assert_array_equal(a.ids, b.ids)
+        np.testing.assert_array_equal(a.dists, b.dists)
+        np.testing.assert_array_equal(a.trace, b.trace)
+
+
+def test_cost_network_ranks_hops(song, setup):
+    """The cost network's prediction correlates with the hop counts it was trained on."""
+    base, q, adj, idx, model = setup
+    r = idx.search(q, 10, 128, learned=3, stop_threshold=2.5, stop_min_hops=8, stop_interval=2)
+    pred = learned.mlp_forward(model.cost_w, model.project(q, 0).astype(np.float32), 16, 1)[:, 0]
+    assert np.corrcoef(pred, np.log2(r.trace[:, 2]))[0, 1] > 0.3
+
+
+def test_nan_cost_keeps_results(song, setup):
+    base, q, adj, idx, model = setup
+    ref = idx.search(q, 10, 128, learned=1)
+    bad = learned.LearnedModel(model.proj_dim, model.mu, model.proj, model.proj_scale, model.entry_w,
+                               model.pivots, model.stop_w, model.dist_scale, np.full_like(model.cost_w, np.nan))
+    bad.set_on(idx)
+    try:
+        r = idx.search(q, 10, 128, learned=5, mode=1)
+    finally:
+        model.set_on(idx)
+    np.testing.assert_array_equal(r.ids, ref.ids)
+    np.testing.assert_array_equal(r.trace, ref.trace)
+
+
+def test_order_needs_cost_network(song, setup):
+    base, q, adj, idx, model = setup
+    plain = learned.LearnedModel(model.proj_dim, model.mu, model.proj, model.proj_scale, model.entry_w,
+                                 model.pivots, model.stop_w, model.dist_scale)
+    plain.set_on(idx)
+    try:
+        with pytest.raises(song.SongError, match="cost network"):
+            idx.search(q[:4], 10, 64, learned=4)
+    finally:
+        model.set_on(idx)
 
 
 def test_trace_labels(song, setup):

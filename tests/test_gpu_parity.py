@@ -59,8 +59,8 @@ def test_config1_exact(song, oracle, c1):
     assert (r.trace[:, 0] == 4).all()
 
 
-@pytest.mark.parametrize("k,ef", [(1, 1), (10, 10), (10, 32), (10, 50), (10, 100), (10, 200),
-                                  (100, 100), (100, 256), (10, 512), (64, 64), (65, 65)])
+@pytest.mark.parametrize("k,ef", [(1, 1), (10, 10), (10, 32), (10, 50), (10, 100), (10, 150), (10, 200),
+                                  (100, 100), (100, 256), (10, 300), (100, 384), (10, 512), (64, 64), (65, 65)])
 def test_queue_sizes(song, oracle, c1, k, ef):
     cfg, adj = c1
     r, o = run_both(song, oracle, cfg["base"], adj, cfg["queries"][:60], k, ef)
@@ -152,6 +152,19 @@ def test_duplicate_ids_in_row(song, oracle, c1):
     adj2[:, 8:] = adj2[:, :8]
     r, o = run_both(song, oracle, cfg["base"], adj2, cfg["queries"][:50], 10, 64)
     assert_same(r, o, "dup rows")
+
+
+@pytest.mark.parametrize("k,ef", [(10, 64), (10, 300), (64, 200), (100, 100)])
+def test_tied_distances(song, oracle, datasets, k, ef):
+    """Every vector three times over: each distance repeats, so the final
+    trim and drain resolve ties by the reference's positional rules."""
+    base, q = datasets.gmm_int(1500, 40, dim=24, n_centres=20, sigma=12.0, seed=11)
+    base = np.repeat(base, 3, axis=0)
+    adj = song.nsw_build(base, block=16, exact_order=False)
+    r, o = run_both(song, oracle, base, adj, q, k, ef)
+    assert_same(r, o, f"ties k={k} ef={ef}")
+    # the test is only meaningful if the k best really do tie
+    assert (np.diff(r.dists[:, :min(k, 3)], axis=1) == 0).any()
 
 
 def test_lane_group_too_small_is_an_error(song, c1):
