@@ -103,7 +103,7 @@ struct Plan {
     int ef, S, G, qch, bloom_words, bloom_shift, n_seeds, batch, persistent;
     int learned;  // bit 0 entry, bit 1 stop, bit 2 cost-ordered queue
     float stop_threshold;
-    int stop_min_hops, stop_interval;
+    int stop_min_hops, stop_interval, stop_max_hops;
     song::LaunchShape shape;
 };
 
@@ -124,8 +124,9 @@ int make_plan(const song_index* ix, int32_t nq, int32_t k, int32_t ef, const son
     p->bloom_shift = ilog2(p->bloom_words);
     p->n_seeds = o->n_seeds ? o->n_seeds : 4;
     if (p->n_seeds < 1 || p->n_seeds > 64) return fail(SONG_ERR_ARG, "n_seeds must be in [1, 64]");
-    const int wpb = o->waves_per_block ? o->waves_per_block : 4;
-    if (wpb != 1 && wpb != 2 && wpb != 4) return fail(SONG_ERR_ARG, "waves_per_block must be 1, 2 or 4");
+    // the learned stop stages its network in each workgroup's LDS: share it over 8 waves
+    const int wpb = o->waves_per_block ? o->waves_per_block : ((o->learned & 2) ? 8 : 4);
+    if (wpb != 1 && wpb != 2 && wpb != 4 && wpb != 8) return fail(SONG_ERR_ARG, "waves_per_block must be 1, 2, 4 or 8");
     p->shape.waves_per_block = wpb;
     if (o->mode != SONG_MODE_PLAIN && o->mode != SONG_MODE_PERSISTENT) return fail(SONG_ERR_ARG, "unknown mode");
     p->persistent = o->mode == SONG_MODE_PERSISTENT;
@@ -157,6 +158,8 @@ int make_plan(const song_index* ix, int32_t nq, int32_t k, int32_t ef, const son
     p->stop_threshold = o->stop_threshold;
     p->stop_min_hops = o->stop_min_hops;
     p->stop_interval = o->stop_interval ? o->stop_interval : 1;
+    p->stop_max_hops = o->stop_max_hops;
+    if (o->stop_max_hops < 0) return fail(SONG_ERR_ARG, "stop_max_hops must be >= 0");
     if (o->learned < 0 || o->learned > 7) return fail(SONG_ERR_ARG, "learned must be 0..7");
     if (p->stop_interval < 1 || p->stop_min_hops < 0) return fail(SONG_ERR_ARG, "stop_interval >= 1, stop_min_hops >= 0");
     if (o->learned) {
@@ -199,6 +202,7 @@ song::SearchArgs make_args(const song_index* ix, const Plan& p, const void* dq, 
         a.L.stop_logit = p.stop_threshold;
         a.L.min_hops = p.stop_min_hops;
         a.L.interval = p.stop_interval;
+        a.L.max_hops = p.stop_max_hops;
         a.L.trace = nullptr;
         a.L.trace_count = nullptr;
         a.L.final_dk = nullptr;

@@ -50,6 +50,7 @@ struct LearnedArgs {
     int32_t use_stop;
     int32_t min_hops;         // first hop at which the stop MLP is consulted
     int32_t interval;         // consult every `interval` hops
+    int32_t max_hops;         // stop after this many expansions (0 = no cap)
     // training traces (all null outside trace mode)
     float* trace;             // [nq x max_evals x (LM_F + 1)]: features, k-th best distance
     int32_t* trace_count;     // [nq]
@@ -83,7 +84,7 @@ struct SearchArgs {
 };
 
 struct LaunchShape {
-    int waves_per_block;     // 1, 2 or 4 queries per workgroup
+    int waves_per_block;     // 1, 2, 4 or 8 queries per workgroup
     int persistent_blocks;   // grid size in persistent mode
 };
 

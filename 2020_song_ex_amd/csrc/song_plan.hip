@@ -168,6 +168,7 @@ __global__ void __launch_bounds__(1024) order_by_cost_kernel(const float* __rest
 __global__ void __launch_bounds__(64) stop_forward_kernel(LearnedArgs L, const float* __restrict__ X, int64_t n,
                                                           float* __restrict__ Y) {
     __shared__ float lds[LM_STOP_FLOATS];
+    __shared__ float hbuf[64];
     stage_stop_weights(L.s_weights, lds);
     __syncthreads();
     const int lane = lane_id();
@@ -175,7 +176,7 @@ __global__ void __launch_bounds__(64) stop_forward_kernel(LearnedArgs L, const f
         float f[LM_F];
 #pragma unroll
         for (int j = 0; j < LM_F; j++) f[j] = X[i * LM_F + j];
-        const float y = stop_logit(lds, f, L.s_b3, lane);
+        const float y = stop_logit(lds, f, L.s_b3, lane, hbuf);
         if (lane == 0) Y[i] = y;
     }
 }

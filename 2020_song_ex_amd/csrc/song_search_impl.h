@@ -195,9 +195,85 @@ __device__ __forceinline__ uint64_t bloom_row(unsigned long long* bloom, uint32_
     return acc;
 }
 
+// The same test-then-set for rows of at most 32 ids (lanes 0..31), with the
+// hashing split over both half-waves: lane j computes hashes 0 and 1 of slot
+// j, lane j + 32 hashes 2 and 3, so each lane runs two of the four 64-bit
+// hash chains.  A slot is "all hit" when both its lanes see their two bits
+// set.  Duplicates (two fresh slots, or two hashes of one slot, on one bit)
+// are replayed slot by slot exactly as in bloom_row.  Returns the mask of
+// fresh slots (bits 0..31).
+__device__ __forceinline__ uint64_t bloom_row_pairs(unsigned long long* bloom, uint32_t wmask, int wshift,
+                                                    uint32_t nbits_mask, int id, bool valid, int lane) {
+    const int half = lane >> 5;
+    const int sid = __shfl(id, lane & 31, 64);
+    const bool sv = __shfl((int)valid, lane & 31, 64) != 0;
+    uint32_t word[2], bitn[2];
+    unsigned long long pre[2];
+    bool miss[2];
+    bool mine = sv;
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+        uint32_t pos = 0u;
+        if (sv) pos = half ? bloom_pos(2 + t, (uint32_t)sid, nbits_mask) : bloom_pos(t, (uint32_t)sid, nbits_mask);
+        word[t] = pos & wmask;
+        bitn[t] = pos >> wshift;
+        pre[t] = sv ? bloom[word[t]] : 0ull;
+        miss[t] = !((pre[t] >> bitn[t]) & 1ull);
+        mine = mine && !miss[t];
+    }
+    const uint64_t hitb = __ballot(mine);
+    const uint32_t all_hit = (uint32_t)hitb & (uint32_t)(hitb >> 32);  // per slot
+    const bool fresh = sv && !((all_hit >> (lane & 31)) & 1u);
+    bool dup = false;
+    if (fresh) {
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+            if (miss[t]) {
+                unsigned long long bit = 1ull << bitn[t];
+                unsigned long long old = atomicOr(&bloom[word[t]], bit);
+                dup = dup || ((old & bit) != 0ull);
+            }
+        }
+    }
+    const uint64_t fresh_mask = __ballot(fresh) & 0xffffffffull;
+    if (__ballot(dup) == 0ull) return fresh_mask;
+
+    if (fresh) {
+#pragma unroll
+        for (int t = 0; t < 2; t++)
+            if (miss[t]) bloom[word[t]] = pre[t];
+    }
+    wave_lds_sync();
+    uint64_t acc = 0ull;
+    uint64_t fm = fresh_mask;
+    while (fm) {
+        const int j = ffs64(fm);
+        fm &= fm - 1ull;
+        const bool me = (lane & 31) == j;
+        bool hit = true;
+        if (me) {
+#pragma unroll
+            for (int t = 0; t < 2; t++) hit = hit && ((bloom[word[t]] >> bitn[t]) & 1ull);
+        }
+        const uint64_t hb = __ballot(me && hit);
+        const bool slot_hit = ((hb >> j) & 1ull) && ((hb >> (j + 32)) & 1ull);
+        if (me && !slot_hit) {
+            // both lanes of the slot may hit one word: atomic, not read-modify-write
+#pragma unroll
+            for (int t = 0; t < 2; t++) atomicOr(&bloom[word[t]], 1ull << bitn[t]);
+        }
+        wave_lds_sync();
+        if (!slot_hit) acc |= 1ull << j;
+    }
+    return acc;
+}
+
 // ---------------------------------------------------------------- the two sets
 
-// Unsorted bounded set of (dist, id), entry i in slot i/64 of lane i%64.
+// Unsorted bounded set of (dist, id): the distance of entry i in VGPR slot
+// i/64 of lane i%64, its id in LDS at ids[i].  Every scan (min, max, first
+// index of a value) runs on the distances; ids are only read or written at one
+// known index, so they cost one LDS access instead of a select per slot.
 // Slots at or beyond sz hold NaN, which fminf / fmaxf ignore and == never
 // matches, so scans need no validity test.  The maximum is cached (wave
 // uniform) and recomputed only after it can have changed.
@@ -206,24 +282,21 @@ struct WSet {
     // ext_vector_type keeps the slots in VGPRs: a plain array gets its
     // per-slot selects folded back into an indexed scratch access.
     typedef float fvec __attribute__((ext_vector_type(S == 1 ? 2 : S)));
-    typedef int ivec __attribute__((ext_vector_type(S == 1 ? 2 : S)));
     fvec d;
-    ivec id;
+    int* ids;  // this wave's LDS array of 64 * S ids
     int sz;    // wave-uniform
     int cap;   // wave-uniform
     float mx;  // cached maximum, valid when mx_ok
     bool mx_ok;
 
-    __device__ __forceinline__ void init(int c) {
+    __device__ __forceinline__ void init(int c, int* lds_ids) {
+        ids = lds_ids;
         sz = 0;
         cap = c < 1 ? 1 : c;
         mx_ok = false;
         mx = 0.f;
 #pragma unroll
-        for (int s = 0; s < S; s++) {
-            d[s] = __builtin_nanf("");
-            id[s] = -1;
-        }
+        for (int s = 0; s < S; s++) d[s] = __builtin_nanf("");
     }
     __device__ __forceinline__ float vmax(int lane) const {
         float m = -INFINITY;
@@ -253,38 +326,38 @@ struct WSet {
         }
         return 0;
     }
-    __device__ __forceinline__ void get(int idx, float& dv, int& iv) const {
+    __device__ __forceinline__ float get_d(int idx) const {
         const int slot = idx >> 6, ln = idx & 63;
         float dd = d[0];
-        int ii = id[0];
 #pragma unroll
         for (int s = 1; s < S; s++)
-            if (slot == s) {
-                dd = d[s];
-                ii = id[s];
-            }
-        dv = readlane_f(dd, ln);
-        iv = __builtin_amdgcn_readlane(ii, ln);
+            if (slot == s) dd = d[s];
+        return readlane_f(dd, ln);
     }
-    // Branch-free: every lane selects, only lane idx%64 in slot idx/64 changes.
-    __device__ __forceinline__ void put(int idx, float dv, int iv, int lane) {
+    __device__ __forceinline__ void get(int idx, float& dv, int& iv) const {
+        dv = get_d(idx);
+        iv = ids[idx];
+    }
+    // Branch-free on the distances: every lane selects, only lane idx%64 in
+    // slot idx/64 changes; one lane writes the id.
+    __device__ __forceinline__ void put_d(int idx, float dv, int lane) {
         const int slot = idx >> 6;
         const bool mine = lane == (idx & 63);
 #pragma unroll
-        for (int s = 0; s < S; s++) {
-            const bool w = mine && slot == s;
-            d[s] = w ? dv : d[s];
-            id[s] = w ? iv : id[s];
-        }
+        for (int s = 0; s < S; s++) d[s] = (mine && slot == s) ? dv : d[s];
+    }
+    __device__ __forceinline__ void put(int idx, float dv, int iv, int lane) {
+        put_d(idx, dv, lane);
+        if (lane == 0) ids[idx] = iv;
     }
     // Remove entry p, filling the hole with the last entry (gpuHeap.h:44-48);
     // the vacated last slot becomes NaN.  The caller keeps mx_ok as fits.
     __device__ __forceinline__ void remove_at(int p, int lane) {
-        float ld;
-        int li;
-        get(sz - 1, ld, li);
+        const int last = sz - 1;
+        const float ld = get_d(last);
+        const int li = ids[last];
         put(p, ld, li, lane);
-        put(sz - 1, __builtin_nanf(""), -1, lane);
+        put_d(last, __builtin_nanf(""), lane);
         sz--;
     }
     // Push items it_d/it_id held by lanes 0..F-1, in lane order, with the
@@ -296,13 +369,10 @@ struct WSet {
         if (a > 0) {
             const int src = (lane - sz) & 63;
             const float md = __shfl(it_d, src, 64);
-            const int mi = __shfl(it_id, src, 64);
             const int slot = src < a ? (sz + src) >> 6 : -1;
 #pragma unroll
-            for (int s = 0; s < S; s++) {
-                d[s] = slot == s ? md : d[s];
-                id[s] = slot == s ? mi : id[s];
-            }
+            for (int s = 0; s < S; s++) d[s] = slot == s ? md : d[s];
+            if (lane < a) ids[sz + lane] = it_id;
             sz += a;
             accepted = a;
             mx_ok = false;
@@ -427,8 +497,11 @@ __device__ __forceinline__ void stage_stop_weights(const float* __restrict__ src
     }
 }
 
-// logit of the stop network for wave-uniform features f[0..7].
-__device__ __forceinline__ float stop_logit(const float* lds, const float (&f)[LM_F], float b3, int lane) {
+// logit of the stop network for wave-uniform features f[0..7].  hbuf: 64
+// floats of this wave's LDS; the hidden layer goes through it so that every
+// lane reads it back as broadcast float4s (no per-unit lane reads).
+__device__ __forceinline__ float stop_logit(const float* lds, const float (&f)[LM_F], float b3, int lane,
+                                            float* hbuf) {
     const float4 wa = *(const float4*)(lds + lane * LM_F);
     const float4 wb = *(const float4*)(lds + lane * LM_F + 4);
     float h = lds[SW_B1 + lane];
@@ -440,16 +513,19 @@ __device__ __forceinline__ float stop_logit(const float* lds, const float (&f)[L
     h = __builtin_fmaf(wb.y, f[5], h);
     h = __builtin_fmaf(wb.z, f[6], h);
     h = __builtin_fmaf(wb.w, f[7], h);
-    h = fmaxf(h, 0.f);
+    hbuf[lane] = fmaxf(h, 0.f);
+    wave_lds_sync();
     float acc = lds[SW_B2 + lane];
 #pragma unroll
     for (int i4 = 0; i4 < 16; i4++) {
         const float4 w = *(const float4*)(lds + SW_W2 + i4 * 256 + lane * 4);
-        acc = __builtin_fmaf(w.x, readlane_f(h, 4 * i4 + 0), acc);
-        acc = __builtin_fmaf(w.y, readlane_f(h, 4 * i4 + 1), acc);
-        acc = __builtin_fmaf(w.z, readlane_f(h, 4 * i4 + 2), acc);
-        acc = __builtin_fmaf(w.w, readlane_f(h, 4 * i4 + 3), acc);
+        const float4 x = *(const float4*)(hbuf + 4 * i4);
+        acc = __builtin_fmaf(w.x, x.x, acc);
+        acc = __builtin_fmaf(w.y, x.y, acc);
+        acc = __builtin_fmaf(w.z, x.z, acc);
+        acc = __builtin_fmaf(w.w, x.w, acc);
     }
+    wave_lds_sync();
     acc = fmaxf(acc, 0.f);
     return wave_sum(acc * lds[SW_W3 + lane]) + b3;
 }
@@ -490,13 +566,14 @@ struct TopK {
 // ---------------------------------------------------------------- one query
 
 // The search of query qid by the calling wave (kg_search_kernel body,
-// kernalGraph.h:192-298).  lds: this wave's bloom[W] | stage_id[64] | stage_d[64].
+// kernalGraph.h:192-298).  lds: this wave's bloom[W] | stage_id[64] | stage_d[64] | set ids.
 template <int METRIC, int S, int G, int QCH, int LEARN>
 __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int lane, unsigned long long* lds,
                                              const float* stop_lds) {
     unsigned long long* bloom = lds;
     int* stage_id = (int*)(bloom + P.bloom_words);
     float* stage_d = (float*)(stage_id + 64);
+    int* set_ids = (int*)(stage_d + 64);  // candidates' ids [64 * S] | results' ids [64 * S]
     const uint32_t wmask = (uint32_t)P.bloom_words - 1u;
     const int wshift = P.bloom_shift;
     const uint32_t nbits_mask = (uint32_t)P.bloom_words * 64u - 1u;
@@ -525,8 +602,8 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
     wave_lds_sync();
 
     WSet<S> cand, res;
-    cand.init(ef);
-    res.init(ef);
+    cand.init(ef, set_ids);
+    res.init(ef, set_ids + 64 * S);
     // QueryTrace, telemetry.h:5-18 (wave-uniform counters)
     // (distance_evals = visited, candidate_nodes = visited - seeds,
     // neighbor_slots = block per expansion that read its row, and the result
@@ -541,12 +618,16 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
     tk.init(k);
     float dk_prev = INFINITY;
     int n_evals = 0;
+    // consulted hops: min_hops, min_hops + interval, ... (hop counts start at
+    // 1, so with min_hops 0 the first is interval)
+    int next_eval = LEARN ? (P.L.min_hops > 0 ? P.L.min_hops : P.L.interval) : 0;
     int last_insert_hop = 0;
 
     // Bloom-test a row of ids (lane order), compute the distances of the new
     // ones and push them to both sets in order.  Returns the number of new ids.
-    auto visit_row = [&](int id, bool valid) __attribute__((always_inline)) -> int {
-        const uint64_t acc = bloom_row(bloom, wmask, wshift, nbits_mask, id, valid);
+    auto visit_row = [&](int id, bool valid, int width) __attribute__((always_inline)) -> int {
+        const uint64_t acc = width <= 32 ? bloom_row_pairs(bloom, wmask, wshift, nbits_mask, id, valid, lane)
+                                         : bloom_row(bloom, wmask, wshift, nbits_mask, id, valid);
         const int F = __popcll(acc);
         if (F == 0) return 0;
         const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(acc >> 32),
@@ -586,7 +667,7 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
                 sid = (int)(((uint64_t)lane * (uint64_t)P.n) / (uint64_t)P.n_entry);
             }
         }
-        const int F = visit_row(sid, sv);
+        const int F = visit_row(sid, sv, P.n_entry);
         tr_seed += F;
         tr_visited += F;
     }
@@ -596,9 +677,7 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
         if (cand.sz == 0) break;
         const float c_min = cand.vmin(lane);
         const int p = cand.first_eq(c_min, lane);
-        float dummy;
-        int cn;
-        cand.get(p, dummy, cn);
+        const int cn = cand.ids[p];
         cand.remove_at(p, lane);  // removing the minimum leaves the maximum unless the set empties
         if (cand.sz == 0) cand.mx_ok = false;
         tr_expanded++;
@@ -610,11 +689,17 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
             }
         }
         if constexpr (LEARN) {
+            if (stopping && P.L.max_hops > 0 && tr_expanded > P.L.max_hops) {
+                tr_early = 3;  // hop cap of the learned termination
+                break;
+            }
             // Learned termination: every `interval` hops from min_hops on, the
             // stop network sees where the search stands; trace mode records
             // the features instead (labels come from the k-th best at the end).
-            if ((tracing || stopping) && tk.dk < INFINITY && tr_expanded >= P.L.min_hops &&
-                (tr_expanded - P.L.min_hops) % P.L.interval == 0) {
+            // next_eval: the next consulted hop (min_hops, then every interval)
+            if (tr_expanded == next_eval) next_eval += P.L.interval;
+            if ((tracing || stopping) && tk.dk < INFINITY && tr_expanded + P.L.interval == next_eval &&
+                tr_expanded >= P.L.min_hops) {
                 const float sc = P.L.dist_scale;
                 const float worst = res.sz == res.cap ? res.cmax(lane) : tk.dk;
                 int closer = 0;  // frontier entries still closer than the k-th best
@@ -644,7 +729,7 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
                     }
                     n_evals++;
                 } else {
-                    const float logit = stop_logit(stop_lds, f, P.L.s_b3, lane);
+                    const float logit = stop_logit(stop_lds, f, P.L.s_b3, lane, stage_d);
                     if (logit > P.L.stop_logit) {  // a NaN logit never stops
                         tr_early = 2;
                         break;
@@ -653,7 +738,7 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
             }
         }
         const int nb = lane < P.block ? P.adj[(size_t)cn * P.block + lane] : -1;
-        tr_visited += visit_row(nb, nb >= 0);
+        tr_visited += visit_row(nb, nb >= 0, P.block);
     }
     const int tr_peakr = res.sz;
 
@@ -687,11 +772,9 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
                 fast = false;
                 break;
             }
-            float dv;
-            int iv;
-            res.get(p, dv, iv);
+            const int iv = res.ids[p];
             if (lane == i) {
-                od = dv;
+                od = m;
                 oi = iv;
             }
             const int slot = p >> 6;
@@ -752,12 +835,14 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
 // ---------------------------------------------------------------- kernels
 
 // LDS of a workgroup: [stop weights when learned] then per wave
-// bloom[W] | stage_id[64] | stage_d[64].
+// bloom[W] | stage_id[64] | stage_d[64] | set ids [2 x 64 * S] (8-byte words).
 constexpr int STOP_LDS_U64 = (LM_STOP_FLOATS + 1) / 2;
+template <int S>
+__host__ __device__ constexpr size_t wave_lds_u64(int bloom_words) { return (size_t)bloom_words + 64 + 64 * S; }
 
 // One wave per query (the reference's one block per query, kernalGraph.h:354).
 template <int METRIC, int S, int G, int QCH, int LEARN>
-__global__ void __launch_bounds__(256) search_kernel(SearchArgs P) {
+__global__ void __launch_bounds__(512) search_kernel(SearchArgs P) {
     extern __shared__ unsigned long long smem[];
     unsigned long long* waves_lds = smem;
     const float* stop_lds = nullptr;
@@ -773,7 +858,7 @@ __global__ void __launch_bounds__(256) search_kernel(SearchArgs P) {
     const int pos = blockIdx.x * (blockDim.x >> 6) + wave;
     if (pos >= P.nq) return;
     const int qid = P.order ? P.order[pos] : pos;
-    search_query<METRIC, S, G, QCH, LEARN>(P, qid, lane_id(), waves_lds + (size_t)wave * (P.bloom_words + 64),
+    search_query<METRIC, S, G, QCH, LEARN>(P, qid, lane_id(), waves_lds + (size_t)wave * wave_lds_u64<S>(P.bloom_words),
                                            stop_lds);
 }
 
@@ -781,7 +866,7 @@ __global__ void __launch_bounds__(256) search_kernel(SearchArgs P) {
 // the next query index from a device counter until all nq are taken.  Every
 // wave leaves once the counter passes nq, so the grid always drains.
 template <int METRIC, int S, int G, int QCH, int LEARN>
-__global__ void __launch_bounds__(256) persistent_kernel(SearchArgs P) {
+__global__ void __launch_bounds__(512) persistent_kernel(SearchArgs P) {
     extern __shared__ unsigned long long smem[];
     unsigned long long* waves_lds = smem;
     const float* stop_lds = nullptr;
@@ -795,7 +880,7 @@ __global__ void __launch_bounds__(256) persistent_kernel(SearchArgs P) {
     }
     const int wave = threadIdx.x >> 6;
     const int lane = lane_id();
-    unsigned long long* lds = waves_lds + (size_t)wave * (P.bloom_words + 64);
+    unsigned long long* lds = waves_lds + (size_t)wave * wave_lds_u64<S>(P.bloom_words);
     for (;;) {
         int item = 0;
         if (lane == 0) item = atomicAdd(P.queue_head, 1);
@@ -814,7 +899,7 @@ static __global__ void zero_counter_kernel(int* c) {
 template <int METRIC, int S, int G, int QCH, int LEARN>
 static hipError_t launch_impl(const SearchArgs& a, const LaunchShape& ls, hipStream_t st) {
     const int threads = 64 * ls.waves_per_block;
-    size_t lds = (size_t)ls.waves_per_block * ((size_t)a.bloom_words + 64) * 8;
+    size_t lds = (size_t)ls.waves_per_block * wave_lds_u64<S>(a.bloom_words) * 8;
     if (LEARN && a.L.use_stop && !a.L.trace) lds += (size_t)STOP_LDS_U64 * 8;
     if (a.queue_head) {
         hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(64), 0, st, a.queue_head);

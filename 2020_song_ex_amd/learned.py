@@ -183,8 +183,11 @@ def _as_rows(vecs, metric, proj_dim):
     return out
 
 
-def kmeans_pivots(index: Index, base, n_pivots=N_PIVOTS, sample=50_000, iters=12, seed=0):
-    """Nodes nearest to k-means centroids of a base sample (distinct ids)."""
+def kmeans_pivots(index: Index, base, n_pivots=N_PIVOTS, sample=50_000, iters=12, seed=0, pool=1.0):
+    """Nodes nearest to k-means centroids of a base sample (distinct ids), chosen among the first
+    pool * n nodes.  The NSW build inserts nodes in id order and early nodes keep the long-range
+    edges of the sparse graph they joined, so a small pool gives seeds from which the greedy
+    search moves fast (node 0, the first reference seed, is the extreme case)."""
     rng = np.random.default_rng(seed)
     n = base.shape[0]
     idx = rng.choice(n, size=min(sample, n), replace=False)
@@ -202,7 +205,13 @@ def kmeans_pivots(index: Index, base, n_pivots=N_PIVOTS, sample=50_000, iters=12
         cq = np.packbits(bits, axis=1, bitorder="little").view(np.uint64)
     else:
         cq = C[:, :base.shape[1]]
-    ids, _, _ = index.bruteforce(cq, 8)
+    n_pool = max(n_pivots * 8, min(n, int(round(pool * n))))
+    if n_pool >= n:
+        ids, _, _ = index.bruteforce(cq, 8)
+    else:
+        sub = Index(base[:n_pool], np.full((n_pool, 2), -1, np.int32), metric=index.metric, device=index.device)
+        ids, _, _ = sub.bruteforce(cq, 8)
+        sub.close()
     chosen = []
     for row in ids:
         for v in row:
@@ -218,7 +227,7 @@ def kmeans_pivots(index: Index, base, n_pivots=N_PIVOTS, sample=50_000, iters=12
 
 def fit(index: Index, base, train_queries, *, k=10, ef=256, n_seeds=4, bloom_bits=0, stop_min_hops=8,
         stop_interval=2, max_evals=128, entry_epochs=20, stop_epochs=4, cost_epochs=20, cost_ef=None,
-        cost_threshold=2.5, seed=0, log=None):
+        cost_threshold=2.5, pivot_pool=1.0, seed=0, log=None):
     """Train the three networks for this index; returns the LearnedModel (already set on the index).
     The cost network learns the hop counts of the learned search (entry + stop at cost_threshold) at
     queue size cost_ef (default ef)."""
@@ -229,7 +238,7 @@ def fit(index: Index, base, train_queries, *, k=10, ef=256, n_seeds=4, bloom_bit
     train_queries = np.ascontiguousarray(train_queries)
     nq = train_queries.shape[0]
 
-    pivots = kmeans_pivots(index, base, seed=seed)
+    pivots = kmeans_pivots(index, base, seed=seed, pool=pivot_pool)
     say(f"pivots: {len(set(pivots.tolist()))} distinct nodes")
     if metric == N.METRIC_HAMMING:
         mu = np.zeros(proj_dim, np.float32)

@@ -3,21 +3,26 @@
 Workload: 1,000,000 x 128 float32 vectors, integer values in [0, 255] from a
 1,000-centre Gaussian mixture (sigma 25), 10,000 held-out queries, NSW graph
 with out-degree 32 (batched CPU build), L2, k = 10.  Queue sizes 32..512 are
-swept (k in {10, 100}); the timed queue is the smallest reaching
-recall@10 >= 0.90 against GPU brute-force ground truth.
+swept (k in {10, 100}) for the record.
 
 A "step" is one search of all 10,000 queries with queries, graph and vectors
-already resident in HBM (song_search_device: one kernel launch).  Two methods
-are timed: the plain wave64 kernel at its smallest queue reaching the target
-recall, and the learned + persistent kernel (entry and stop networks trained
-on held-out draws of the same mixture, launch and threshold autotuned on a
-validation set, then timed on the 10,000 bench queries).  value = queries per
-second of the learned + persistent method when it reaches the target recall
-on the bench queries, else of the plain method.  The JSON line also carries the
-roofline of the search kernel (algorithmic bytes from the run's own trace
-counters / HIP-event kernel time) and the reference's CPU search
-(Graph::search, graph.h:33-115, compiled from its sources into oracle/_ref)
-timed on this host on a bounded sample.
+already resident in HBM (song_search_device: one planning pass when learned,
+one search launch).  Two methods are tuned on 10,000 further held-out draws of
+the mixture and then timed on the bench queries:
+  * plain -- the reference algorithm on the wave64 kernel, one wave per query
+    in input order (kernalGraph.h:184-299 semantics, bit-exact to the
+    oracle): the queue size and workgroup shape with the highest throughput
+    whose validation recall@10 reaches the target;
+  * learned + persistent -- the persistent work-queue kernel with the learned
+    stop network (and, where the tuner picks them, the cost-ordered queue and
+    the hop cap), networks trained on 15,000 held-out draws.
+value = queries per second of the learned + persistent method when its
+recall@10 on the bench queries reaches the target, else of the plain method;
+both are in the JSON line with their ratio.  The line also carries the
+roofline of the plain search kernel (algorithmic bytes from the run's own
+trace counters / HIP-event kernel time), Poisson-load serving latency (p50 /
+p99) and the reference's CPU search (Graph::search, graph.h:33-115, compiled
+from its sources into oracle/_ref) timed on this host on a bounded sample.
 """
 import argparse
 import importlib
@@ -69,11 +74,12 @@ def main():
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
-    ap.add_argument("--train", type=int, default=20000, help="training queries for the learned search")
-    ap.add_argument("--valid", type=int, default=5000, help="validation queries for the autotuner")
+    ap.add_argument("--train", type=int, default=15000, help="training queries for the learned search")
+    ap.add_argument("--valid", type=int, default=10000, help="validation queries for the autotuner")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-load", type=float, default=0.5, help="Poisson rate as a fraction of batch QPS")
-    ap.add_argument("--serve-batch", type=int, default=512)
+    ap.add_argument("--serve-batch", type=int, default=1024)
+    ap.add_argument("--serve-streams", type=int, default=8)
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run a plain copy kernel of this many MiB (byte-counter calibration)")
     args = ap.parse_args()
@@ -173,7 +179,7 @@ def main():
     ef = args.ef
     plain_params = {"waves_per_block": 4}
     if not ef and extra is not None:
-        grid = {"ef": [256, 320, 384, 416, 448, 480, 512], "mode": [0], "waves_per_block": [1, 4]}
+        grid = {"ef": [320, 336, 352, 368, 384, 416, 448, 512], "mode": [0], "waves_per_block": [1, 4]}
         res = autotune.tune(idx, valid, vgt, args.k, grid, tune_target, log=log)
         plain_tuned = res.table()
         if res.best is not None:
@@ -193,7 +199,7 @@ def main():
 
     abytes = algorithmic_bytes(trace, dim, block, args.k)
     achieved = abytes / (plain_kernel_ms / 1e3) / 1e9
-    slots = 1 if ef <= 64 else 2 if ef <= 128 else 4 if ef <= 256 else 8
+    slots = 1 if ef <= 64 else 2 if ef <= 128 else 3 if ef <= 192 else 4 if ef <= 256 else 6 if ef <= 384 else 8
     kname = f"search_kernel<0,{slots},8,4,0>"
     # HBM bytes per launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE (calibrated
     # against a copy kernel) for this exact kernel and workload, when recorded
@@ -208,6 +214,10 @@ def main():
                 "kernel": kname, "kernel_avg_ms": round(plain_kernel_ms, 4),
                 "algorithmic_bytes_per_launch": abytes}
 
+    # ---------------- persistent kernel alone (no learning), for the record
+    pers_qps, _, pers_kernel_ms = timed(args.k, ef, mode=1)
+    log(f"persistent (no learning): queue {ef}, {pers_qps:.0f} queries/s")
+
     # ---------------- learned + persistent
     learned_out = None
     serve_out = None
@@ -215,12 +225,12 @@ def main():
     if not args.no_learned:
         learned = importlib.import_module("2020_song_ex_amd.learned")
         t0 = time.time()
-        model = learned.fit(idx, base, train, k=args.k, ef=512, stop_min_hops=16, stop_interval=4,
-                            max_evals=192, entry_epochs=20, stop_epochs=4, log=log)
+        model = learned.fit(idx, base, train, k=args.k, ef=384, stop_min_hops=16, stop_interval=4,
+                            max_evals=192, entry_epochs=20, stop_epochs=4, cost_threshold=3.0, log=log)
         log(f"learned model trained in {time.time() - t0:.1f}s")
-        grid = {"ef": [384, 448, 512], "learned": [3], "stop_threshold": [1.0, 1.5, 2.0, 2.5, 3.0, 4.0],
-                "stop_min_hops": [16], "stop_interval": [4], "mode": [1], "waves_per_cu": [16, 32],
-                "waves_per_block": [4]}
+        grid = {"ef": [352, 384, 448], "learned": [2, 6], "stop_threshold": [2.5, 3.0, 3.5, 4.0],
+                "stop_max_hops": [0, 500, 600, 700], "stop_min_hops": [16], "stop_interval": [4], "mode": [1],
+                "waves_per_block": [8]}
         res = autotune.tune(idx, valid, vgt, args.k, grid, tune_target, log=log)
         if res.best is not None:
             p = dict(res.best.params)
@@ -233,6 +243,7 @@ def main():
                            "ms_per_step": round(lms_step, 4), "kernel_ms": round(lkernel_ms, 4),
                            "expanded_mean": float(ltr[:, 2].mean()),
                            "learned_stops": float((ltr[:, 11] == 2).mean()),
+                           "hop_capped": float((ltr[:, 11] == 3).mean()),
                            "speedup_vs_plain": round(lqps / plain_qps, 3),
                            "validation_recall": round(res.best.recall, 4), "tuned": res.table()}
             log(f"learned+persistent: {learned_out['params']} recall@10 {lrec:.4f}, {lqps:.0f} queries/s "
@@ -244,10 +255,10 @@ def main():
     if not args.no_serve:
         serving = importlib.import_module("2020_song_ex_amd.serving")
         sef, sp = best_search if best_search else (ef, {"mode": 0, **plain_params})
-        sp = {kk: v for kk, v in sp.items() if kk != "waves_per_block"}
         cap_qps = learned_out["qps"] if best_search else plain_qps
         rate = args.serve_load * cap_qps
-        sv = serving.Server(idx, max_batch=args.serve_batch, n_streams=4, k_max=args.k, ef_max=512, **sp)
+        sv = serving.Server(idx, max_batch=args.serve_batch, n_streams=args.serve_streams, k_max=args.k,
+                            ef_max=512, **sp)
         ks = np.full(nq, args.k, np.int32)
         efs_req = np.full(nq, sef, np.int32)
         sids, _, lat, wall = sv.run_poisson(queries, ks, efs_req, rate, seed=11)
@@ -257,7 +268,8 @@ def main():
                      "p50_ms": round(float(np.percentile(lat, 50)), 4),
                      "p99_ms": round(float(np.percentile(lat, 99)), 4),
                      "max_ms": round(float(lat.max()), 4), "recall_at_10": round(recall_at(sids, gt, 10), 4),
-                     "queue": sef, "search": sp, "launches": st["launches"], "max_batch": args.serve_batch}
+                     "queue": sef, "search": sp, "launches": st["launches"], "max_batch": args.serve_batch,
+                     "streams": args.serve_streams}
         log(f"serving {serve_out}")
 
     # ---------------- CPU baseline: the reference's own Graph::search
@@ -292,8 +304,12 @@ def main():
         "config": {"workload": f"SONG search, {args.n}x{dim} integer GMM (1000 centres, sigma 25), "
                                f"{nq} queries, NSW out-degree {block}, L2, k {args.k}",
                    "method": "learned+persistent" if use_learned else "plain",
-                   "plain": {"queue": ef, "recall_at_10": round(plain_rec, 4), "qps": round(plain_qps, 1),
-                             "kernel_ms": round(plain_kernel_ms, 4)},
+                   "plain": {"queue": ef, **plain_params, "recall_at_10": round(plain_rec, 4),
+                             "qps": round(plain_qps, 1), "kernel_ms": round(plain_kernel_ms, 4)},
+                   "persistent_no_learning": {"queue": ef, "qps": round(pers_qps, 1),
+                                              "kernel_ms": round(pers_kernel_ms, 4)},
+                   "learned_vs_plain": (round(learned_out["qps"] / plain_qps, 3)
+                                        if learned_out and "qps" in learned_out else None),
                    "learned_persistent": learned_out, "plain_tuned": plain_tuned},
         "latency": serve_out,
         "roofline": roofline,

@@ -103,7 +103,9 @@ typedef struct song_search_opts {
                                 (N_MULTIQUERY, config.h:38, is 512) */
     int32_t n_seeds;         /* evenly spaced entry points, 1..64;
                                 0 = 4 (N_MULTIPROBE, config.h:34) */
-    int32_t waves_per_block; /* 1, 2 or 4 queries per workgroup; 0 = 4 */
+    int32_t waves_per_block; /* 1, 2, 4 or 8 queries per workgroup; 0 = 4
+                                (8 with the learned stop, whose network each
+                                workgroup stages in LDS) */
     int32_t lanes_per_row;   /* lanes that share one vector row (power of
                                 two, 2..64); 0 = chosen from dim.  Fixes the
                                 float summation order (xor butterfly over
@@ -125,7 +127,9 @@ typedef struct song_search_opts {
     float stop_threshold;    /* logit threshold of the stop network */
     int32_t stop_min_hops;   /* first hop at which it is consulted */
     int32_t stop_interval;   /* consult every this many hops (>= 1) */
-    int32_t reserved[5];
+    int32_t stop_max_hops;   /* with learned bit 1: also stop once a query has
+                                expanded this many nodes (0 = no cap) */
+    int32_t reserved[4];
 } song_search_opts;
 
 enum { SONG_MODE_PLAIN = 0, SONG_MODE_PERSISTENT = 1 };

@@ -360,6 +360,8 @@ typedef struct {
     int n_seeds;
     int exact_visited; /* knob: exact visited set instead of the Bloom filter */
     int unbounded;     /* knob: frontier never evicts (cap = n + n_seeds)     */
+    int max_hops;      /* learned search's hop cap (0 = none; no reference
+                          counterpart): stop once expanded > max_hops        */
 } or_params;
 
 /* Seeds at the entry of the search, kernalGraph.h:215-231: n_seeds evenly
@@ -434,6 +436,10 @@ static void search_one(const or_space *sp, const or_params *p, or_work *w, const
             tr[TR_EARLY] = 1;
             break;
         }
+        if (p->max_hops > 0 && tr[TR_EXPANDED] > p->max_hops) {
+            tr[TR_EARLY] = 3;
+            break;
+        }
         for (int j = 0; j < p->block; j++) {
             tr[TR_SLOTS]++;
             int nb = p->adj[(int64_t)cn * p->block + j];
@@ -471,16 +477,33 @@ static int64_t row_elems(const or_space *sp) { return sp->metric == OR_METRIC_HA
  *          NULL uses the reference seeds.
  * Returns 0, or -1 on bad arguments.
  */
+int or_search_capped(const void *data, const int32_t *adj, int64_t n, int dim, int block, int metric,
+                     const void *queries, int nq, int k, int ef, int64_t bloom_bits, int n_seeds, int G,
+                     int ref_order, int exact_visited, int unbounded, const int32_t *entries, int n_entry,
+                     int max_hops, int32_t *ids_out, float *dists_out, int32_t *trace_out);
+
 int or_search(const void *data, const int32_t *adj, int64_t n, int dim, int block, int metric,
               const void *queries, int nq, int k, int ef, int64_t bloom_bits, int n_seeds, int G,
               int ref_order, int exact_visited, int unbounded, const int32_t *entries, int n_entry,
               int32_t *ids_out, float *dists_out, int32_t *trace_out)
 {
+    return or_search_capped(data, adj, n, dim, block, metric, queries, nq, k, ef, bloom_bits, n_seeds, G,
+                            ref_order, exact_visited, unbounded, entries, n_entry, 0, ids_out, dists_out,
+                            trace_out);
+}
+
+/* or_search with the learned search's hop cap (max_hops > 0). */
+int or_search_capped(const void *data, const int32_t *adj, int64_t n, int dim, int block, int metric,
+                     const void *queries, int nq, int k, int ef, int64_t bloom_bits, int n_seeds, int G,
+                     int ref_order, int exact_visited, int unbounded, const int32_t *entries, int n_entry,
+                     int max_hops, int32_t *ids_out, float *dists_out, int32_t *trace_out)
+{
+    if (max_hops < 0) return -1;
     if (n < 1 || k < 1 || ef < 1 || block < 1 || nq < 0 || G < 1 || G > 64) return -1;
     if (!exact_visited && (bloom_bits < 64 || (bloom_bits % 64) != 0)) return -1;
     if (ef < k) ef = k; /* kernalGraph.h:315 (the reference also caps ef at 50; lifted here) */
     or_space sp = {metric, dim, G, ref_order, (const float *)data, (const uint64_t *)data};
-    or_params p = {adj, n, block, k, ef, bloom_bits / 64, n_seeds, exact_visited, unbounded};
+    or_params p = {adj, n, block, k, ef, bloom_bits / 64, n_seeds, exact_visited, unbounded, max_hops};
     int ne = entries ? n_entry : n_seeds;
     int64_t fcap = unbounded ? n + ne + 1 : ef;
     or_work w;

@@ -158,6 +158,19 @@ def test_order_needs_cost_network(song, setup):
         model.set_on(idx)
 
 
+@pytest.mark.parametrize("cap", [1, 20, 60])
+def test_hop_cap_matches_oracle(song, oracle, setup, cap):
+    """The learned termination's hop cap, with the stop network never firing,
+    equals the oracle's capped search (early_stopped = 3 where it cut)."""
+    base, q, adj, idx, model = setup
+    r = idx.search(q, 10, 128, learned=2, stop_threshold=1e30, stop_min_hops=1, stop_max_hops=cap)
+    oi, od, ot = oracle.search(base, adj, q, 10, 128, block=32, bloom_bits=1 << 14, group=4, max_hops=cap)
+    np.testing.assert_array_equal(r.ids, oi)
+    np.testing.assert_array_equal(r.dists.view(np.uint32), od.view(np.uint32))
+    np.testing.assert_array_equal(r.trace, ot)
+    assert (r.trace[:, 11] == 3).any()
+
+
 def test_trace_labels(song, setup):
     base, q, adj, idx, model = setup
     N = song._native

@@ -139,7 +139,7 @@ def test_isolated_nodes_and_seeds(song, oracle, c1):
 def test_batches_and_blocks(song, oracle, c1):
     """Results do not depend on batch size or queries per workgroup."""
     cfg, adj = c1
-    for wpb, batch in [(1, 0), (2, 7), (4, 33), (4, 512)]:
+    for wpb, batch in [(1, 0), (2, 7), (4, 33), (4, 512), (8, 0), (8, 9)]:
         r, o = run_both(song, oracle, cfg["base"], adj, cfg["queries"], 10, 64,
                         waves_per_block=wpb, batch=batch)
         assert_same(r, o, f"wpb={wpb} batch={batch}")
@@ -181,7 +181,7 @@ def test_persistent_equals_plain(song, oracle, c1):
     idx = song.Index(cfg["base"], adj)
     plain = idx.search(cfg["queries"], 10, 64)
     for wpc in (1, 3, 8, 32):
-        for wpb in (1, 4):
+        for wpb in (1, 4, 8):
             r = idx.search(cfg["queries"], 10, 64, mode=1, waves_per_cu=wpc, waves_per_block=wpb)
             np.testing.assert_array_equal(r.ids, plain.ids)
             np.testing.assert_array_equal(r.dists, plain.dists)

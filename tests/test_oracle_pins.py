@@ -86,6 +86,29 @@ def test_trace_invariants(oracle, song, datasets):
         np.testing.assert_array_equal(exact.astype(np.float32), d)
 
 
+@pytest.mark.parametrize("cap", [1, 7, 40])
+def test_hop_cap(oracle, song, datasets, cap):
+    """The learned search's hop cap: a query that ends within the cap is untouched;
+    a capped one stops at expansion cap + 1 (early_stopped = 3) holding the best
+    of what it saw, and its counters obey the same relations."""
+    cfg = datasets.config1()
+    adj = song.nsw_build(cfg["base"], block=16, exact_order=False)
+    full = oracle.search(cfg["base"], adj, cfg["queries"], 10, 64, block=16, bloom_bits=1 << 15, group=8)
+    ids, d, tr = oracle.search(cfg["base"], adj, cfg["queries"], 10, 64, block=16, bloom_bits=1 << 15,
+                               group=8, max_hops=cap)
+    short = full[2][:, 2] <= cap
+    np.testing.assert_array_equal(ids[short], full[0][short])
+    np.testing.assert_array_equal(tr[short], full[2][short])
+    cut = ~short
+    assert cut.any()
+    assert (tr[cut, 11] == 3).all() and (tr[cut, 2] == cap + 1).all()
+    assert (tr[:, 3] == 16 * (tr[:, 2] - (tr[:, 11] > 0))).all()
+    assert (tr[:, 5] == tr[:, 1]).all()
+    assert (np.diff(d, axis=1) >= 0).all()
+    # fewer expansions can only leave the k-th best distance as good or worse
+    assert (d[:, 9] >= full[1][:, 9]).all()
+
+
 # ---------------------------------------------------------------- Bloom filter
 
 MASK64 = (1 << 64) - 1
