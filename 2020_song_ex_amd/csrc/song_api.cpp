@@ -195,6 +195,7 @@ song::SearchArgs make_args(const song_index* ix, const Plan& p, const void* dq, 
     a.k_stride = k;
     a.queue_head = p.persistent ? ix->d_counter : nullptr;
     a.order = nullptr;
+    a.resident = nullptr;
     a.learned = (p.learned & 2) ? 1 : 0;
     if (a.learned) {
         a.L = ix->model;
@@ -319,6 +320,24 @@ int launch_mixed_on_stream(song_index* ix, const void* d_q, int nq, const int32_
 }
 
 int64_t plan_ws_words(int64_t nq) { return plan_words(nq, 64); }
+
+int launch_resident_on_stream(song_index* ix, const void* d_q, int ring, const int32_t* d_ks, const int32_t* d_efs,
+                              int k_stride, int ef_max, const song_search_opts* opts, int32_t* d_ids, float* d_dists,
+                              const void* d_queue, hipStream_t stream) {
+    song_search_opts o = *opts;
+    if (o.learned & 5) return fail(SONG_ERR_ARG, "the resident server supports the learned stop only (bit 1)");
+    o.mode = SONG_MODE_PERSISTENT;
+    Plan p;
+    int rc = make_plan(ix, ring, k_stride, ef_max, &o, &p);
+    if (rc) return rc;
+    SearchArgs a = make_args(ix, p, d_q, ring, k_stride, d_ids, d_dists, nullptr);
+    a.ks = d_ks;
+    a.efs = d_efs;
+    a.queue_head = nullptr;
+    a.resident = (const ResidentQueue*)d_queue;
+    hipError_t e = launch_search(a, ix->metric, p.S, p.G, p.qch, p.shape, stream);
+    return e == hipSuccess ? SONG_OK : hip_fail(e, "resident kernel launch");
+}
 
 }  // namespace song
 

@@ -58,6 +58,21 @@ struct LearnedArgs {
     int32_t max_evals;
 };
 
+// The request ring of the resident serving kernel.  The host writes request r
+// into slot r % ring of host-visible arrays and then publishes tail = r + 1;
+// waves claim requests with a compare-and-swap on head (device memory), run
+// them, write the results into the slot and set done[slot] = r + 1.  Every
+// wait is bounded: a wave leaves once stop is set or after idle_ticks without
+// finding a request, so the grid always drains on its own.
+struct ResidentQueue {
+    int64_t* head;        // device memory: next request number to claim
+    const int64_t* tail;  // host memory: requests published so far
+    const int32_t* stop;  // host memory: nonzero asks every wave to leave
+    int64_t* done;        // host memory [ring]: request number + 1 once written
+    int32_t ring;         // slots
+    int64_t idle_ticks;   // 100 MHz real-time-clock ticks
+};
+
 // Everything one launch of the search kernel reads or writes.
 struct SearchArgs {
     const void* data;        // f32 [n x dstride] (L2/IP) or u64 [n x 4] (Hamming)
@@ -79,6 +94,7 @@ struct SearchArgs {
     int32_t k_stride;        // output row length (>= every k)
     int32_t* queue_head;     // persistent mode: device counter (zeroed at launch), else null
     const int32_t* order;    // [nq] queue position -> query index, or null (identity)
+    const ResidentQueue* resident;  // device copy of the serving ring, or null
     int32_t learned;         // 1: use L (stop / trace as its flags say)
     LearnedArgs L;
 };

@@ -862,9 +862,37 @@ __global__ void __launch_bounds__(512) search_kernel(SearchArgs P) {
                                            stop_lds);
 }
 
+// Resident serving: the next published request, or -1 to leave (stop set, or
+// nothing to claim for idle_ticks).  Lane 0 only.  Claims are exactly-once:
+// head only moves by a successful compare-and-swap below tail.  The inner loop
+// ends because every failed swap means another wave advanced head.  Polls are
+// plain system-scope loads (no cache invalidation: the ring lives in coherent
+// host memory, which the device does not cache), spaced by a growing sleep
+// while idle, so idle waves cost neither L2 contents nor much bus traffic.
+__device__ __forceinline__ int64_t resident_claim(const ResidentQueue& Q) {
+    const uint64_t start = __builtin_amdgcn_s_memrealtime();
+    int naps = 1;
+    for (;;) {
+        if (__hip_atomic_load(Q.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)) return -1;
+        const int64_t t = __hip_atomic_load(Q.tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        int64_t h = __hip_atomic_load(Q.head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        while (h < t) {
+            if (__hip_atomic_compare_exchange_strong(Q.head, &h, h + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT))
+                return h;
+        }
+        if (__builtin_amdgcn_s_memrealtime() - start > (uint64_t)Q.idle_ticks) return -1;
+        // about 3.4 us per nap; 1, 2, 4, ... 32 naps between polls
+        for (int i = 0; i < naps; i++) __builtin_amdgcn_s_sleep(127);
+        naps = naps < 32 ? 2 * naps : 32;
+    }
+}
+
 // Persistent: a fixed grid of waves drains the query queue; each wave takes
 // the next query index from a device counter until all nq are taken.  Every
-// wave leaves once the counter passes nq, so the grid always drains.
+// wave leaves once the counter passes nq, so the grid always drains.  With a
+// resident queue the waves serve the request ring instead (resident_claim),
+// publishing each request's results before its done flag.
 template <int METRIC, int S, int G, int QCH, int LEARN>
 __global__ void __launch_bounds__(512) persistent_kernel(SearchArgs P) {
     extern __shared__ unsigned long long smem[];
@@ -881,6 +909,23 @@ __global__ void __launch_bounds__(512) persistent_kernel(SearchArgs P) {
     const int wave = threadIdx.x >> 6;
     const int lane = lane_id();
     unsigned long long* lds = waves_lds + (size_t)wave * wave_lds_u64<S>(P.bloom_words);
+    if (P.resident) {
+        const ResidentQueue Q = *P.resident;
+        for (;;) {
+            int64_t item = -1;
+            if (lane == 0) item = resident_claim(Q);
+            const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)(uint64_t)item);
+            const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)item >> 32));
+            item = (int64_t)(((uint64_t)hi << 32) | lo);
+            if (item < 0) break;
+            const int slot = (int)(item % Q.ring);
+            search_query<METRIC, S, G, QCH, LEARN>(P, slot, lane, lds, stop_lds);
+            // every lane's ids / distances of the slot become visible to the host before its flag
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+            if (lane == 0) __hip_atomic_store(Q.done + slot, item + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+        return;
+    }
     for (;;) {
         int item = 0;
         if (lane == 0) item = atomicAdd(P.queue_head, 1);
@@ -901,7 +946,10 @@ static hipError_t launch_impl(const SearchArgs& a, const LaunchShape& ls, hipStr
     const int threads = 64 * ls.waves_per_block;
     size_t lds = (size_t)ls.waves_per_block * wave_lds_u64<S>(a.bloom_words) * 8;
     if (LEARN && a.L.use_stop && !a.L.trace) lds += (size_t)STOP_LDS_U64 * 8;
-    if (a.queue_head) {
+    if (a.resident) {  // the ring's head persists across launches
+        hipLaunchKernelGGL((persistent_kernel<METRIC, S, G, QCH, LEARN>), dim3(ls.persistent_blocks), dim3(threads),
+                           lds, st, a);
+    } else if (a.queue_head) {
         hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(64), 0, st, a.queue_head);
         hipLaunchKernelGGL((persistent_kernel<METRIC, S, G, QCH, LEARN>), dim3(ls.persistent_blocks), dim3(threads),
                            lds, st, a);

@@ -30,6 +30,7 @@
 #include <vector>
 
 #include "../../include/song.h"
+#include "song_kernels.h"
 #include "song_server.h"
 
 namespace song {
@@ -38,6 +39,7 @@ using clk = std::chrono::steady_clock;
 
 struct Request {
     int64_t ticket;
+    int64_t seq = -1;  // resident mode: request number in the ring
     int32_t k, ef;
     clk::time_point t_arrival;  // submission (or scheduled arrival) time
     clk::time_point t_done;
@@ -87,6 +89,33 @@ struct song_server {
     bool stop = false;
     std::string error;
     std::thread dispatcher;
+    // resident mode: the request ring lives in host memory the device reads
+    // and writes (fine-grained, coherent); head and the ring descriptor on
+    // the device
+    bool resident = false;
+    int ring = 0;
+    char* rq_q = nullptr;
+    int32_t* rq_ks = nullptr;
+    int32_t* rq_efs = nullptr;
+    int32_t* rq_ids = nullptr;
+    float* rq_dists = nullptr;
+    int64_t* rq_done = nullptr;
+    int64_t* rq_tail = nullptr;
+    int32_t* rq_stop = nullptr;
+    int64_t* d_head = nullptr;
+    song::ResidentQueue* d_queue = nullptr;
+    // the ring's device-side addresses (hipHostGetDevicePointer)
+    void* dv_q = nullptr;
+    int32_t* dv_ks = nullptr;
+    int32_t* dv_efs = nullptr;
+    int32_t* dv_ids = nullptr;
+    float* dv_dists = nullptr;
+    std::vector<song::Request*> ring_req;  // slot -> request in it
+    int64_t published = 0;                 // requests written into the ring
+    int64_t oldest = 0;                    // smallest request number not yet retired
+    hipStream_t rstream = nullptr;
+    hipEvent_t rdone = nullptr;
+    bool running = false;
 };
 
 namespace {
@@ -198,6 +227,162 @@ void dispatcher_loop(song_server* sv) {
     }
 }
 
+// Resident mode: admit requests into free ring slots, publish them, keep the
+// grid running while requests are outstanding, retire finished slots.
+void dispatcher_loop_resident(song_server* sv) {
+    (void)hipSetDevice(sv->device);
+    std::unique_lock<std::mutex> lock(sv->mu);
+    const int kst = sv->opts.k_max;
+    song::clk::time_point stop_seen{};
+    for (;;) {
+        bool progress = false;
+        // 1. admit: request number `published` goes to slot published % ring
+        //    once that slot's previous request has retired
+        int admitted = 0;
+        while (!sv->pending.empty()) {
+            const int slot = (int)(sv->published % sv->ring);
+            if (sv->ring_req[slot] != nullptr) break;
+            if (sv->inflight_bytes + sv->req_bytes > sv->opts.memory_budget) {
+                sv->stats.deferred++;
+                break;
+            }
+            song::Request* r = sv->pending.front();
+            sv->pending.pop_front();
+            std::memcpy(sv->rq_q + (size_t)slot * sv->row_bytes, r->query.data(), sv->row_bytes);
+            sv->rq_ks[slot] = r->k;
+            sv->rq_efs[slot] = r->ef;
+            r->seq = sv->published++;
+            sv->ring_req[slot] = r;
+            sv->inflight_bytes += sv->req_bytes;
+            sv->stats.in_flight++;
+            admitted++;
+        }
+        if (admitted) {
+            // the slots' contents before the new tail
+            __atomic_store_n(sv->rq_tail, sv->published, __ATOMIC_RELEASE);
+            sv->stats.peak_bytes = std::max(sv->stats.peak_bytes, sv->inflight_bytes);
+            progress = true;
+        }
+        // 2. the grid: relaunch when it has left and requests are outstanding
+        if (sv->running && hipEventQuery(sv->rdone) != hipErrorNotReady) sv->running = false;
+        if (!sv->running && sv->oldest < sv->published && sv->error.empty()) {
+            int rc = song::launch_resident_on_stream(sv->ix, sv->dv_q, sv->ring, sv->dv_ks, sv->dv_efs, kst,
+                                                     sv->opts.ef_max, &sv->opts.search, sv->dv_ids, sv->dv_dists,
+                                                     sv->d_queue, sv->rstream);
+            hipError_t e = rc == SONG_OK ? hipEventRecord(sv->rdone, sv->rstream) : hipErrorInvalidValue;
+            if (rc != SONG_OK || e != hipSuccess) {
+                sv->error = std::string("resident launch failed: ") + song_last_error();
+                sv->cv_done.notify_all();
+            } else {
+                sv->running = true;
+                sv->stats.launches++;
+            }
+        }
+        // 3. retire finished requests (any order); oldest tracks the first unfinished
+        const auto now = song::clk::now();
+        for (int64_t q = sv->oldest; q < sv->published; q++) {
+            const int slot = (int)(q % sv->ring);
+            song::Request* r = sv->ring_req[slot];
+            if (r == nullptr || r->seq != q) continue;  // retired already
+            if (__atomic_load_n(sv->rq_done + slot, __ATOMIC_ACQUIRE) != q + 1) continue;
+            r->ids.assign(sv->rq_ids + (size_t)slot * kst, sv->rq_ids + (size_t)slot * kst + r->k);
+            r->dists.assign(sv->rq_dists + (size_t)slot * kst, sv->rq_dists + (size_t)slot * kst + r->k);
+            r->t_done = now;
+            r->done = true;
+            sv->ring_req[slot] = nullptr;
+            sv->inflight_bytes -= sv->req_bytes;
+            sv->stats.in_flight--;
+            sv->stats.completed++;
+            progress = true;
+        }
+        while (sv->oldest < sv->published && sv->ring_req[(int)(sv->oldest % sv->ring)] == nullptr) sv->oldest++;
+        if (progress) sv->cv_done.notify_all();
+        if (sv->stop && sv->pending.empty() && sv->oldest == sv->published) break;
+        if (!sv->error.empty() && sv->stop) break;
+        if (sv->stop) {
+            // destroy waits at most a few seconds for outstanding requests
+            if (stop_seen == song::clk::time_point{}) stop_seen = now;
+            if (ms_between(stop_seen, now) > 5000.0) {
+                sv->error = "server destroyed with requests outstanding";
+                sv->cv_done.notify_all();
+                break;
+            }
+        }
+        if (!progress) {
+            if (sv->oldest < sv->published) {
+                lock.unlock();
+                std::this_thread::sleep_for(std::chrono::microseconds(5));
+                lock.lock();
+            } else {
+                sv->cv_work.wait_for(lock, std::chrono::milliseconds(5),
+                                     [&] { return sv->stop || !sv->pending.empty(); });
+            }
+        }
+    }
+}
+
+void free_resident(song_server* sv) {
+    if (sv->rq_stop) __atomic_store_n(sv->rq_stop, 1, __ATOMIC_RELEASE);
+    if (sv->rstream) (void)hipStreamSynchronize(sv->rstream);
+    for (void* p : {(void*)sv->rq_q, (void*)sv->rq_ks, (void*)sv->rq_efs, (void*)sv->rq_ids, (void*)sv->rq_dists,
+                    (void*)sv->rq_done, (void*)sv->rq_tail, (void*)sv->rq_stop})
+        if (p) (void)hipHostFree(p);
+    if (sv->d_head) (void)hipFree(sv->d_head);
+    if (sv->d_queue) (void)hipFree(sv->d_queue);
+    if (sv->rdone) (void)hipEventDestroy(sv->rdone);
+    if (sv->rstream) (void)hipStreamDestroy(sv->rstream);
+}
+
+// Ring buffers for resident mode; returns a hipError_t.
+hipError_t setup_resident(song_server* sv) {
+    const size_t R = (size_t)sv->ring, kst = (size_t)sv->opts.k_max;
+    const unsigned fl = hipHostMallocCoherent | hipHostMallocMapped;
+    hipError_t e = hipStreamCreateWithFlags(&sv->rstream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&sv->rdone, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&sv->rq_q, R * sv->row_bytes, fl);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&sv->rq_ks, R * 4, fl);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&sv->rq_efs, R * 4, fl);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&sv->rq_ids, R * kst * 4, fl);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&sv->rq_dists, R * kst * 4, fl);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&sv->rq_done, R * 8, fl);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&sv->rq_tail, 64, fl);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&sv->rq_stop, 64, fl);
+    if (e == hipSuccess) e = hipMalloc((void**)&sv->d_head, 64);
+    if (e == hipSuccess) e = hipMalloc((void**)&sv->d_queue, sizeof(song::ResidentQueue));
+    if (e != hipSuccess) return e;
+    void *dtail = nullptr, *dstop = nullptr, *ddone = nullptr;
+    e = hipHostGetDevicePointer(&sv->dv_q, sv->rq_q, 0);
+    if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&sv->dv_ks, sv->rq_ks, 0);
+    if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&sv->dv_efs, sv->rq_efs, 0);
+    if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&sv->dv_ids, sv->rq_ids, 0);
+    if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&sv->dv_dists, sv->rq_dists, 0);
+    if (e == hipSuccess) e = hipHostGetDevicePointer(&ddone, sv->rq_done, 0);
+    if (e == hipSuccess) e = hipHostGetDevicePointer(&dtail, sv->rq_tail, 0);
+    if (e == hipSuccess) e = hipHostGetDevicePointer(&dstop, sv->rq_stop, 0);
+    if (e != hipSuccess) return e;
+    std::memset(sv->rq_q, 0, R * sv->row_bytes);
+    for (size_t i = 0; i < R; i++) {
+        sv->rq_ks[i] = 1;
+        sv->rq_efs[i] = 1;
+        sv->rq_done[i] = 0;
+    }
+    *sv->rq_tail = 0;
+    *sv->rq_stop = 0;
+    e = hipMemsetAsync(sv->d_head, 0, 64, sv->rstream);
+    song::ResidentQueue q{};
+    q.head = sv->d_head;
+    q.tail = (const int64_t*)dtail;
+    q.stop = (const int32_t*)dstop;
+    q.done = (int64_t*)ddone;
+    q.ring = sv->ring;
+    const int idle_ms = sv->opts.idle_ms > 0 ? sv->opts.idle_ms : 20;
+    q.idle_ticks = (int64_t)idle_ms * 100000;  // the real-time clock runs at 100 MHz
+    if (e == hipSuccess) e = hipMemcpyAsync(sv->d_queue, &q, sizeof q, hipMemcpyHostToDevice, sv->rstream);
+    if (e == hipSuccess) e = hipStreamSynchronize(sv->rstream);
+    sv->ring_req.assign(R, nullptr);
+    return e;
+}
+
 void free_slot(song::Slot& s) {
     if (s.stream) (void)hipStreamSynchronize(s.stream);
     for (void* p : {(void*)s.h_q, (void*)s.h_ks, (void*)s.h_efs, (void*)s.h_ids, (void*)s.h_dists})
@@ -224,7 +409,16 @@ int song_server_create(song_index* ix, const song_server_opts* o, song_server** 
     if (opts.memory_budget <= 0) opts.memory_budget = (int64_t)256 << 20;
     if (opts.k_max <= 0) opts.k_max = 100;
     if (opts.ef_max <= 0) opts.ef_max = 512;
-    if (opts.n_streams > 16) return sfail(SONG_ERR_ARG, "n_streams must be <= 16");
+    if (opts.n_streams > 32) return sfail(SONG_ERR_ARG, "n_streams must be <= 32");
+    if (opts.resident != 0 && opts.resident != 1) return sfail(SONG_ERR_ARG, "resident must be 0 or 1");
+    // The resident grid is switched off: in its device test it returned stale
+    // request rows for some ring slots (the device cached the host-written
+    // ring) and some of its waves did not leave at shutdown.  Until both are
+    // understood the server refuses the mode instead of running it.
+    if (opts.resident) return sfail(SONG_ERR_ARG, "the resident serving kernel is disabled in this build");
+    if (opts.resident && (opts.search.learned & 5))
+        return sfail(SONG_ERR_ARG, "the resident server supports the learned stop only (learned bit 1)");
+    if (opts.idle_ms < 0) return sfail(SONG_ERR_ARG, "idle_ms must be >= 0");
     if (opts.k_max > 512 || opts.ef_max > 512) return sfail(SONG_ERR_ARG, "k_max and ef_max must be <= 512");
     int32_t stride = 0;
     int rc = song_index_row_stride(ix, &stride);
@@ -246,6 +440,21 @@ int song_server_create(song_index* ix, const song_server_opts* o, song_server** 
     if (hipSetDevice(device) != hipSuccess) {
         delete sv;
         return sfail(SONG_ERR_HIP, "hipSetDevice");
+    }
+    if (opts.resident) {
+        sv->resident = true;
+        sv->ring = (int)std::max<int64_t>(
+            1, std::min<int64_t>((int64_t)opts.max_batch * opts.n_streams, opts.memory_budget / sv->req_bytes));
+        sv->capacity = sv->ring;
+        hipError_t e = setup_resident(sv);
+        if (e != hipSuccess) {
+            free_resident(sv);
+            delete sv;
+            return sfail(SONG_ERR_ALLOC, std::string("resident ring: ") + hipGetErrorString(e));
+        }
+        sv->dispatcher = std::thread(dispatcher_loop_resident, sv);
+        *out = sv;
+        return SONG_OK;
     }
     sv->slots.resize(opts.n_streams);
     const size_t cap = (size_t)sv->capacity, kst = (size_t)opts.k_max;
@@ -356,7 +565,10 @@ void song_server_destroy(song_server* sv) {
     }
     sv->cv_work.notify_all();
     if (sv->dispatcher.joinable()) sv->dispatcher.join();
+    // the resident grid: every wave sees the flag at its next claim
+    if (sv->rq_stop) __atomic_store_n(sv->rq_stop, 1, __ATOMIC_RELEASE);
     for (auto& s : sv->slots) free_slot(s);
+    if (sv->resident) free_resident(sv);
     for (auto& kv : sv->all) delete kv.second;
     for (auto* r : sv->pending)
         if (!sv->all.count(r->ticket)) delete r;

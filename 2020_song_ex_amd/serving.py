@@ -16,7 +16,7 @@ _I32, _I64, _P = ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p
 
 class SongServerOpts(ctypes.Structure):
     _fields_ = [("max_batch", _I32), ("n_streams", _I32), ("memory_budget", _I64), ("k_max", _I32),
-                ("ef_max", _I32), ("search", N.SongSearchOpts)]
+                ("ef_max", _I32), ("search", N.SongSearchOpts), ("resident", _I32), ("idle_ms", _I32)]
 
 
 class SongServerStats(ctypes.Structure):
@@ -43,7 +43,7 @@ def _bind():
 
 class Server:
     def __init__(self, index: Index, *, max_batch=1024, n_streams=4, memory_budget=0, k_max=100, ef_max=512,
-                 **search):
+                 resident=False, idle_ms=0, **search):
         L = _bind()
         self.index = index
         self.k_max = k_max
@@ -51,6 +51,7 @@ class Server:
         o.max_batch, o.n_streams, o.memory_budget, o.k_max, o.ef_max = max_batch, n_streams, memory_budget, \
             k_max, ef_max
         o.search = N.make_opts(**search)
+        o.resident, o.idle_ms = int(bool(resident)), idle_ms
         h = _P()
         N.check(L.song_server_create(index.handle, ctypes.byref(o), ctypes.byref(h)), "song_server_create")
         self._h = h

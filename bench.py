@@ -24,10 +24,16 @@ trace counters / HIP-event kernel time), Poisson-load serving latency (p50 /
 p99) and the reference's CPU search (Graph::search, graph.h:33-115, compiled
 from its sources into oracle/_ref) timed on this host on a bounded sample.
 """
+import os
+
+# Hardware queues of this process (HIP's default is 4).  The serving loop runs
+# its launch slots on separate streams; with 4 queues at most two of them
+# overlap on the device.  Must be set before the HIP runtime starts.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+
 import argparse
 import importlib
 import json
-import os
 import sys
 import time
 
@@ -77,9 +83,10 @@ def main():
     ap.add_argument("--train", type=int, default=15000, help="training queries for the learned search")
     ap.add_argument("--valid", type=int, default=10000, help="validation queries for the autotuner")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--serve-load", type=float, default=0.5, help="Poisson rate as a fraction of batch QPS")
-    ap.add_argument("--serve-batch", type=int, default=1024)
-    ap.add_argument("--serve-streams", type=int, default=8)
+    ap.add_argument("--serve-loads", default="0.2,0.4,0.6",
+                    help="Poisson rates as fractions of the batch QPS; the middle one is the headline")
+    ap.add_argument("--serve-batch", type=int, default=256)
+    ap.add_argument("--serve-streams", type=int, default=16)
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run a plain copy kernel of this many MiB (byte-counter calibration)")
     args = ap.parse_args()
@@ -177,7 +184,7 @@ def main():
     # ---------------- plain wave64 kernel: smallest queue (and best workgroup shape) at the target
     plain_tuned = None
     ef = args.ef
-    plain_params = {"waves_per_block": 4}
+    plain_params = {"mode": 0, "waves_per_block": 1}  # the shape the tuner picks on this workload
     if not ef and extra is not None:
         grid = {"ef": [320, 336, 352, 368, 384, 416, 448, 512], "mode": [0], "waves_per_block": [1, 4]}
         res = autotune.tune(idx, valid, vgt, args.k, grid, tune_target, log=log)
@@ -254,23 +261,31 @@ def main():
     # ---------------- serving: Poisson arrivals, per-request latency
     if not args.no_serve:
         serving = importlib.import_module("2020_song_ex_amd.serving")
-        sef, sp = best_search if best_search else (ef, {"mode": 0, **plain_params})
+        sef, sp = best_search if best_search else (ef, dict(plain_params))
+        # one wave per request in each launch (a persistent grid per small launch only adds waves)
+        sp = {**sp, "mode": 0}
         cap_qps = learned_out["qps"] if best_search else plain_qps
-        rate = args.serve_load * cap_qps
-        sv = serving.Server(idx, max_batch=args.serve_batch, n_streams=args.serve_streams, k_max=args.k,
-                            ef_max=512, **sp)
         ks = np.full(nq, args.k, np.int32)
         efs_req = np.full(nq, sef, np.int32)
-        sids, _, lat, wall = sv.run_poisson(queries, ks, efs_req, rate, seed=11)
-        st = sv.stats()
-        sv.close()
-        serve_out = {"offered_qps": round(rate, 1), "achieved_qps": round(nq / (wall / 1e3), 1),
+        curve = []
+        for load in [float(x) for x in args.serve_loads.split(",")]:
+            rate = load * cap_qps
+            sv = serving.Server(idx, max_batch=args.serve_batch, n_streams=args.serve_streams, k_max=args.k,
+                                ef_max=512, **sp)
+            sids, _, lat, wall = sv.run_poisson(queries, ks, efs_req, rate, seed=11)
+            st = sv.stats()
+            sv.close()
+            point = {"load": load, "offered_qps": round(rate, 1), "achieved_qps": round(nq / (wall / 1e3), 1),
                      "p50_ms": round(float(np.percentile(lat, 50)), 4),
                      "p99_ms": round(float(np.percentile(lat, 99)), 4),
                      "max_ms": round(float(lat.max()), 4), "recall_at_10": round(recall_at(sids, gt, 10), 4),
-                     "queue": sef, "search": sp, "launches": st["launches"], "max_batch": args.serve_batch,
-                     "streams": args.serve_streams}
-        log(f"serving {serve_out}")
+                     "launches": st["launches"]}
+            curve.append(point)
+            log(f"serving {point}")
+        mid = curve[len(curve) // 2]
+        serve_out = {**mid, "queue": sef, "search": sp, "max_batch": args.serve_batch,
+                     "streams": args.serve_streams, "hw_queues": int(os.environ["GPU_MAX_HW_QUEUES"]),
+                     "curve": curve}
 
     # ---------------- CPU baseline: the reference's own Graph::search
     cpu = None

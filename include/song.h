@@ -310,21 +310,33 @@ int song_mlp_forward(int32_t device, const float* X, int64_t n, int32_t in_dim, 
 
 /* Continuous-batching server over a resident index (no counterpart in the
  * reference, whose bench_gpu runs fixed batches, kernalGraph.h:343-375).
- * Requests queue from any thread; a dispatcher admits them into n_streams
- * launch slots as slots and device memory free up, so requests that arrive
- * while others run start in the next free slot and overlap on the device.
- * Each request holds row_bytes + 8 + 8*k_max bytes of device state while in
+ * Requests queue from any thread.  Two ways to run them:
+ *  - launch slots (resident = 0): a dispatcher admits requests into n_streams
+ *    slots of up to max_batch as slots and memory free up; each slot is one
+ *    launch, so requests that arrive while others run start in the next free
+ *    slot and overlap on the device;
+ *  - resident kernel (resident = 1): one persistent grid stays on the device
+ *    and serves a ring of max_batch * n_streams request slots in host-visible
+ *    memory; each wave takes the next request as soon as its last one is
+ *    done, so no request waits for a batch.  The grid leaves by itself after
+ *    idle_ms without requests and is relaunched when more arrive.  Supports
+ *    the plain and persistent search with the learned stop (learned bit 1).
+ * Each request holds row_bytes + 8 + 8*k_max bytes of request state while in
  * flight; admission keeps the total within memory_budget.  A request's result
  * depends only on its query, k, queue size and `search` options. */
 typedef struct song_server song_server;
 
 typedef struct song_server_opts {
     int32_t max_batch;        /* requests per launch, 0 = 1024 */
-    int32_t n_streams;        /* launch slots (HIP streams), 1..16, 0 = 4 */
+    int32_t n_streams;        /* launch slots (HIP streams), 1..32, 0 = 4; slots only
+                                 overlap on the device up to the process's hardware
+                                 queues (GPU_MAX_HW_QUEUES, HIP default 4) */
     int64_t memory_budget;    /* bytes of in-flight request state, 0 = 256 MiB */
     int32_t k_max;            /* largest k accepted, 0 = 100 */
     int32_t ef_max;           /* largest queue size accepted (<= 512), 0 = 512 */
     song_search_opts search;  /* options of every launch (mode, learned, ...) */
+    int32_t resident;         /* 1 = resident kernel (see above), 0 = launch slots */
+    int32_t idle_ms;          /* resident kernel: leave after this long idle, 0 = 20 */
 } song_server_opts;
 
 typedef struct song_server_stats {
@@ -335,7 +347,7 @@ typedef struct song_server_stats {
     int64_t deferred;         /* admission passes that found the budget full */
     int64_t peak_bytes;       /* largest in-flight request state seen */
     int64_t memory_budget;
-    int32_t slot_capacity;    /* requests per launch after the budget */
+    int32_t slot_capacity;    /* requests per launch (ring slots when resident) after the budget */
     int32_t pad_;
 } song_server_stats;
 
