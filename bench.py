@@ -29,7 +29,7 @@ import os
 # Hardware queues of this process (HIP's default is 4).  The serving loop runs
 # its launch slots on separate streams; with 4 queues at most two of them
 # overlap on the device.  Must be set before the HIP runtime starts.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+os.environ["GPU_MAX_HW_QUEUES"] = "16"
 
 import argparse
 import importlib
