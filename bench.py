@@ -19,8 +19,8 @@ the mixture and then timed on the bench queries:
 value = queries per second of the learned + persistent method when its
 recall@10 on the bench queries reaches the target, else of the plain method;
 both are in the JSON line with their ratio.  The line also carries the
-roofline of the plain search kernel (algorithmic bytes from the run's own
-trace counters / HIP-event kernel time), Poisson-load serving latency (p50 /
+roofline of the search kernel behind `value` (algorithmic bytes from the run's
+own trace counters / HIP-event kernel time), Poisson-load serving latency (p50 /
 p99) and the reference's CPU search (Graph::search, graph.h:33-115, compiled
 from its sources into oracle/_ref) timed on this host on a bounded sample.
 """
@@ -80,6 +80,11 @@ def main():
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--model", default="", help="load the learned model from this file if it exists, "
+                                                 "else train it and save it there")
+    ap.add_argument("--learned-config", default="",
+                    help='JSON of a learned + persistent configuration to time instead of tuning, '
+                         'e.g. {"ef": 384, "learned": 6, "stop_threshold": 3.5, "stop_max_hops": 600, ...}')
     ap.add_argument("--train", type=int, default=15000, help="training queries for the learned search")
     ap.add_argument("--valid", type=int, default=10000, help="validation queries for the autotuner")
     ap.add_argument("--no-serve", action="store_true")
@@ -204,22 +209,35 @@ def main():
     plain_qps, plain_ms_step, plain_kernel_ms = timed(args.k, ef, **plain_params)
     log(f"plain: queue {ef} {plain_params}, recall@10 {plain_rec:.4f}, {plain_qps:.0f} queries/s")
 
-    abytes = algorithmic_bytes(trace, dim, block, args.k)
-    achieved = abytes / (plain_kernel_ms / 1e3) / 1e9
-    slots = 1 if ef <= 64 else 2 if ef <= 128 else 3 if ef <= 192 else 4 if ef <= 256 else 6 if ef <= 384 else 8
-    kname = f"search_kernel<0,{slots},8,4,0>"
-    # HBM bytes per launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE (calibrated
-    # against a copy kernel) for this exact kernel and workload, when recorded
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tpath):
-        trec = json.load(open(tpath)).get(f"{kname} {args.n}x{dim} nq={nq} queue={ef} k={args.k}")
-        if trec:
-            traffic = trec["bytes_per_launch"]
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": kname, "kernel_avg_ms": round(plain_kernel_ms, 4),
-                "algorithmic_bytes_per_launch": abytes}
+    def slots_for(q):
+        return 1 if q <= 64 else 2 if q <= 128 else 3 if q <= 192 else 4 if q <= 256 else 6 if q <= 384 else 8
+
+    def roofline_of(trace_rows, kernel_ms, kname, key):
+        """Algorithmic bytes of one launch / its HIP-event duration; HBM traffic per launch
+        from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE record of this exact kernel and
+        workload, when there is one."""
+        abytes = algorithmic_bytes(trace_rows, dim, block, args.k)
+        achieved = abytes / (kernel_ms / 1e3) / 1e9
+        traffic = None
+        tpath = os.path.join(ROOT, "profiles", "traffic.json")
+        if os.path.exists(tpath):
+            trec = json.load(open(tpath)).get(key)
+            if trec:
+                traffic = trec["bytes_per_launch"]
+        return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kname,
+                "kernel_avg_ms": round(kernel_ms, 4), "algorithmic_bytes_per_launch": abytes}
+
+    pname = f"search_kernel<0,{slots_for(ef)},8,4,0>"
+    plain_roofline = roofline_of(trace, plain_kernel_ms, pname,
+                                 f"{pname} {args.n}x{dim} nq={nq} queue={ef} k={args.k}")
+
+    # ---------------- the host-buffer path (song_search: queries H2D, ids D2H), for the record
+    t0 = time.perf_counter()
+    for _ in range(3):
+        idx.search(queries, args.k, ef, dists=False, trace=False, **plain_params)
+    pcie_qps = 3 * nq / (time.perf_counter() - t0)
+    log(f"plain through host buffers (PCIe included): {pcie_qps:.0f} queries/s")
 
     # ---------------- persistent kernel alone (no learning), for the record
     pers_qps, _, pers_kernel_ms = timed(args.k, ef, mode=1)
@@ -227,18 +245,30 @@ def main():
 
     # ---------------- learned + persistent
     learned_out = None
+    learned_roofline = None
     serve_out = None
     best_search = None
     if not args.no_learned:
         learned = importlib.import_module("2020_song_ex_amd.learned")
         t0 = time.time()
-        model = learned.fit(idx, base, train, k=args.k, ef=384, stop_min_hops=16, stop_interval=4,
-                            max_evals=192, entry_epochs=20, stop_epochs=4, cost_threshold=3.0, log=log)
-        log(f"learned model trained in {time.time() - t0:.1f}s")
+        if args.model and os.path.exists(args.model):
+            model = learned.LearnedModel.load(args.model)
+            model.set_on(idx)
+            log(f"learned model loaded from {args.model}")
+        else:
+            model = learned.fit(idx, base, train, k=args.k, ef=384, stop_min_hops=16, stop_interval=4,
+                                max_evals=192, entry_epochs=20, stop_epochs=4, cost_threshold=3.0, log=log)
+            log(f"learned model trained in {time.time() - t0:.1f}s")
+            if args.model:
+                model.save(args.model)
         grid = {"ef": [352, 384, 448], "learned": [2, 6], "stop_threshold": [2.5, 3.0, 3.5, 4.0],
                 "stop_max_hops": [0, 500, 600, 700], "stop_min_hops": [16], "stop_interval": [4], "mode": [1],
                 "waves_per_block": [8]}
+        if args.learned_config:
+            grid = {kk: [v] for kk, v in json.loads(args.learned_config).items()}
         res = autotune.tune(idx, valid, vgt, args.k, grid, tune_target, log=log)
+        if res.best is None and args.learned_config:
+            res.best = res.tried[0]  # time the given configuration whatever its validation recall
         if res.best is not None:
             p = dict(res.best.params)
             lef = p.pop("ef")
@@ -246,6 +276,9 @@ def main():
             lrec = recall_at(ids, gt, 10)
             lqps, lms_step, lkernel_ms = timed(args.k, lef, **p)
             best_search = (lef, p)
+            lname = f"persistent_kernel<0,{slots_for(lef)},8,4,1>"
+            lkey = f"{lname} {args.n}x{dim} nq={nq} " + " ".join(f"{kk}={v}" for kk, v in sorted({"ef": lef, **p}.items()))
+            learned_roofline = roofline_of(ltr, lkernel_ms, lname, lkey)
             learned_out = {"params": {"ef": lef, **p}, "recall_at_10": round(lrec, 4), "qps": round(lqps, 1),
                            "ms_per_step": round(lms_step, 4), "kernel_ms": round(lkernel_ms, 4),
                            "expanded_mean": float(ltr[:, 2].mean()),
@@ -309,6 +342,8 @@ def main():
 
     use_learned = (learned_out is not None and "qps" in learned_out
                    and learned_out["recall_at_10"] >= args.target_recall)
+    # the roofline of the kernel behind `value`; the other method's in config
+    roofline = learned_roofline if use_learned else plain_roofline
     value = learned_out["qps"] if use_learned else plain_qps
     ms_step = learned_out["ms_per_step"] if use_learned else plain_ms_step
     out = {
@@ -321,10 +356,12 @@ def main():
                    "method": "learned+persistent" if use_learned else "plain",
                    "plain": {"queue": ef, **plain_params, "recall_at_10": round(plain_rec, 4),
                              "qps": round(plain_qps, 1), "kernel_ms": round(plain_kernel_ms, 4)},
+                   "plain_host_buffers_qps": round(pcie_qps, 1),
                    "persistent_no_learning": {"queue": ef, "qps": round(pers_qps, 1),
                                               "kernel_ms": round(pers_kernel_ms, 4)},
                    "learned_vs_plain": (round(learned_out["qps"] / plain_qps, 3)
                                         if learned_out and "qps" in learned_out else None),
+                   "plain_roofline": plain_roofline, "learned_roofline": learned_roofline,
                    "learned_persistent": learned_out, "plain_tuned": plain_tuned},
         "latency": serve_out,
         "roofline": roofline,
