@@ -324,6 +324,7 @@ int64_t plan_ws_words(int64_t nq) { return plan_words(nq, 64); }
 int launch_resident_on_stream(song_index* ix, const void* d_q, int ring, const int32_t* d_ks, const int32_t* d_efs,
                               int k_stride, int ef_max, const song_search_opts* opts, int32_t* d_ids, float* d_dists,
                               const void* d_queue, hipStream_t stream) {
+    return fail(SONG_ERR_ARG, "the resident serving kernel is disabled in this build");
     song_search_opts o = *opts;
     if (o.learned & 5) return fail(SONG_ERR_ARG, "the resident server supports the learned stop only (bit 1)");
     o.mode = SONG_MODE_PERSISTENT;
@@ -699,7 +700,7 @@ int song_index_set_model(song_index* ix, const song_learned_model* m) {
     put(7, m->entry_w3);
     put(8, m->entry_b3);
     std::memcpy(host.data() + offs[9], m->pivots, LM_C * 4);
-    put(10, m->stop_weights);
+    song::stop_weights_to_kernel_layout(m->stop_weights, host.data() + offs[10]);
     put(11, m->cost_weights);
     std::lock_guard<std::mutex> lock(ix->mu);
     HIP_TRY(hipSetDevice(ix->device), "hipSetDevice");
@@ -862,7 +863,9 @@ int song_mlp_forward(int32_t device, const float* X, int64_t n, int32_t in_dim, 
     if (e == hipSuccess) e = hipMalloc(&dW, (size_t)(total + 4) * 4);
     if (e == hipSuccess) e = hipMalloc(&dY, (size_t)n * out_dim * 4);
     if (e == hipSuccess) e = hipMemcpy(dX, X, (size_t)n * in_dim * 4, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(dW, weights, (size_t)total * 4, hipMemcpyHostToDevice);
+    std::vector<float> wk(weights, weights + total);
+    if (in_dim == 8) song::stop_weights_to_kernel_layout(weights, wk.data());
+    if (e == hipSuccess) e = hipMemcpy(dW, wk.data(), (size_t)total * 4, hipMemcpyHostToDevice);
     song::LearnedArgs L{};
     song::PlanArgs P{};
     int which = 0;

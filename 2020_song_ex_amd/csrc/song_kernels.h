@@ -17,6 +17,16 @@ constexpr int LM_STOP_FLOATS = LM_H * LM_F + LM_H + LM_H * LM_H + LM_H + LM_H;  
 
 constexpr int LM_COST_FLOATS = LM_H * LM_P + LM_H + LM_H * LM_H + LM_H + LM_H + 1;  // ... w3 b3
 
+// Stop-network weights (w1 [64 x 8], b1, w2 [64 x 64] row-major, b2, w3) into
+// the search kernel's layout: w2 interleaved so that element (j, i) sits at
+// w2 + (i / 4) * 256 + j * 4 + i % 4 (lane j reads coalesced float4s).
+inline void stop_weights_to_kernel_layout(const float* src, float* dst) {
+    const int w2 = LM_H * LM_F + LM_H;
+    for (int i = 0; i < LM_STOP_FLOATS; i++) dst[i] = src[i];
+    for (int j = 0; j < LM_H; j++)
+        for (int i = 0; i < LM_H; i++) dst[w2 + (i / 4) * 256 + j * 4 + (i % 4)] = src[w2 + j * LM_H + i];
+}
+
 // The query-side networks (entry, cost), run by the planning kernel before
 // the search: z = scale * (proj (q - mu)) feeds both.
 struct PlanArgs {

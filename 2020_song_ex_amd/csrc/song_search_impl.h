@@ -477,24 +477,18 @@ __device__ __forceinline__ void hamming_distances(const ulonglong2* __restrict__
 
 // ---------------------------------------------------------------- learned entry / termination
 
-// The stop network's weights, staged once per workgroup in LDS: w1 [64][8],
-// b1 [64], w2 interleaved so lane j reads 16 conflict-free float4
-// (element (j, i) at (i/4)*256 + j*4 + i%4), b2 [64], w3 [64].
-// Offsets of the stop network's arrays in s_weights (and in LDS).
+// The stop network's weights in the kernel's layout (the host converts them
+// once, stop_weights_to_kernel_layout): w1 [64][8], b1 [64], w2 interleaved so
+// lane j reads 16 conflict-free, coalesced float4 (element (j, i) at
+// (i/4)*256 + j*4 + i%4), b2 [64], w3 [64].  Each workgroup stages them in
+// LDS once.
 constexpr int SW_B1 = LM_H * LM_F;         // b1 [64]
-constexpr int SW_W2 = SW_B1 + LM_H;        // w2 [64][64]
+constexpr int SW_W2 = SW_B1 + LM_H;        // w2 [64][64], interleaved
 constexpr int SW_B2 = SW_W2 + LM_H * LM_H; // b2 [64]
 constexpr int SW_W3 = SW_B2 + LM_H;        // w3 [64]
 
 __device__ __forceinline__ void stage_stop_weights(const float* __restrict__ src, float* lds) {
-    for (int i = threadIdx.x; i < LM_STOP_FLOATS; i += blockDim.x) {
-        int dst = i;
-        if (i >= SW_W2 && i < SW_B2) {
-            const int e = i - SW_W2, j = e / LM_H, c = e % LM_H;
-            dst = SW_W2 + (c >> 2) * 256 + j * 4 + (c & 3);
-        }
-        lds[dst] = src[i];
-    }
+    for (int i = threadIdx.x; i < LM_STOP_FLOATS; i += blockDim.x) lds[i] = src[i];
 }
 
 // logit of the stop network for wave-uniform features f[0..7].  hbuf: 64
@@ -862,6 +856,12 @@ __global__ void __launch_bounds__(512) search_kernel(SearchArgs P) {
                                            stop_lds);
 }
 
+// The resident serving grid (a request ring in host memory, claimed wave by
+// wave) is compiled out: in its device run some claimed requests read stale
+// ring rows and some waves did not leave at shutdown.  The host refuses the
+// mode while this is false.
+constexpr bool kResidentServing = false;
+
 // Resident serving: the next published request, or -1 to leave (stop set, or
 // nothing to claim for idle_ticks).  Lane 0 only.  Claims are exactly-once:
 // head only moves by a successful compare-and-swap below tail.  The inner loop
@@ -909,7 +909,8 @@ __global__ void __launch_bounds__(512) persistent_kernel(SearchArgs P) {
     const int wave = threadIdx.x >> 6;
     const int lane = lane_id();
     unsigned long long* lds = waves_lds + (size_t)wave * wave_lds_u64<S>(P.bloom_words);
-    if (P.resident) {
+    if constexpr (kResidentServing) {
+      if (P.resident) {
         const ResidentQueue Q = *P.resident;
         for (;;) {
             int64_t item = -1;
@@ -925,6 +926,7 @@ __global__ void __launch_bounds__(512) persistent_kernel(SearchArgs P) {
             if (lane == 0) __hip_atomic_store(Q.done + slot, item + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         }
         return;
+      }
     }
     for (;;) {
         int item = 0;
@@ -946,7 +948,7 @@ static hipError_t launch_impl(const SearchArgs& a, const LaunchShape& ls, hipStr
     const int threads = 64 * ls.waves_per_block;
     size_t lds = (size_t)ls.waves_per_block * wave_lds_u64<S>(a.bloom_words) * 8;
     if (LEARN && a.L.use_stop && !a.L.trace) lds += (size_t)STOP_LDS_U64 * 8;
-    if (a.resident) {  // the ring's head persists across launches
+    if (kResidentServing && a.resident) {  // the ring's head persists across launches
         hipLaunchKernelGGL((persistent_kernel<METRIC, S, G, QCH, LEARN>), dim3(ls.persistent_blocks), dim3(threads),
                            lds, st, a);
     } else if (a.queue_head) {

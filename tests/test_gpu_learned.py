@@ -95,6 +95,20 @@ def test_stop_never_fires_equals_plain(song, setup):
     np.testing.assert_array_equal(r.trace, plain.trace)
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_stop_results_do_not_depend_on_workgroup_shape(song, setup, mode):
+    """Every workgroup stages the stop network in its LDS; 1 and 8 waves per
+    workgroup give the same results."""
+    base, q, adj, idx, model = setup
+    kw = dict(learned=2, stop_threshold=2.0, stop_min_hops=8, stop_interval=2, mode=mode)
+    a = idx.search(q, 10, 128, waves_per_block=1, **kw)
+    b = idx.search(q, 10, 128, waves_per_block=8, **kw)
+    np.testing.assert_array_equal(a.ids, b.ids)
+    np.testing.assert_array_equal(a.dists, b.dists)
+    np.testing.assert_array_equal(a.trace, b.trace)
+    assert (a.trace[:, 11] == 2).any()
+
+
 def test_learned_entry_matches_oracle_with_those_seeds(song, oracle, setup):
     """With learned entry only, the search equals the oracle run from the same seed list."""
     base, q, adj, idx, model = setup
