@@ -39,7 +39,7 @@ class SongSearchOpts(ctypes.Structure):
                 ("waves_per_block", _I32), ("lanes_per_row", _I32), ("mode", _I32),
                 ("waves_per_cu", _I32), ("learned", _I32), ("stop_threshold", ctypes.c_float),
                 ("stop_min_hops", _I32), ("stop_interval", _I32), ("stop_max_hops", _I32),
-                ("reserved", _I32 * 4)]
+                ("stop_confirm", _I32), ("reserved", _I32 * 3)]
 
 
 EXPORTS = (
@@ -119,13 +119,14 @@ MODE_PLAIN, MODE_PERSISTENT = 0, 1
 
 def make_opts(bloom_bits=0, batch=0, n_seeds=0, waves_per_block=0, lanes_per_row=0, mode=0,
               waves_per_cu=0, learned=0, stop_threshold=0.0, stop_min_hops=0, stop_interval=1,
-              stop_max_hops=0):
+              stop_max_hops=0, stop_confirm=0):
     o = SongSearchOpts()
     o.learned = learned
     o.stop_threshold = stop_threshold
     o.stop_min_hops = stop_min_hops
     o.stop_interval = stop_interval
     o.stop_max_hops = stop_max_hops
+    o.stop_confirm = stop_confirm
     o.mode = mode
     o.waves_per_cu = waves_per_cu
     o.bloom_bits = bloom_bits

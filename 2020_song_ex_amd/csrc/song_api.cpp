@@ -103,7 +103,7 @@ struct Plan {
     int ef, S, G, qch, bloom_words, bloom_shift, n_seeds, batch, persistent;
     int learned;  // bit 0 entry, bit 1 stop, bit 2 cost-ordered queue
     float stop_threshold;
-    int stop_min_hops, stop_interval, stop_max_hops;
+    int stop_min_hops, stop_interval, stop_max_hops, stop_confirm;
     song::LaunchShape shape;
 };
 
@@ -116,7 +116,7 @@ int make_plan(const song_index* ix, int32_t nq, int32_t k, int32_t ef, const son
     if (ef < k) ef = k;  // kernalGraph.h:315
     if (ef > 512) return fail(SONG_ERR_ARG, "ef (queue size) must be <= 512 after ef = max(ef, k)");
     p->ef = ef;
-    p->S = ef <= 64 ? 1 : ef <= 128 ? 2 : ef <= 192 ? 3 : ef <= 256 ? 4 : ef <= 384 ? 6 : 8;
+    p->S = ef <= 64 ? 1 : ef <= 128 ? 2 : ef <= 192 ? 3 : ef <= 256 ? 4 : ef <= 384 ? 6 : ef <= 448 ? 7 : 8;
     int64_t bits = o->bloom_bits ? o->bloom_bits : 16384;
     if (!is_pow2(bits) || bits < 512 || bits > 65536)
         return fail(SONG_ERR_ARG, "bloom_bits must be a power of two in [512, 65536]");
@@ -160,6 +160,8 @@ int make_plan(const song_index* ix, int32_t nq, int32_t k, int32_t ef, const son
     p->stop_interval = o->stop_interval ? o->stop_interval : 1;
     p->stop_max_hops = o->stop_max_hops;
     if (o->stop_max_hops < 0) return fail(SONG_ERR_ARG, "stop_max_hops must be >= 0");
+    p->stop_confirm = o->stop_confirm ? o->stop_confirm : 1;
+    if (o->stop_confirm < 0 || o->stop_confirm > 64) return fail(SONG_ERR_ARG, "stop_confirm must be in [0, 64]");
     if (o->learned < 0 || o->learned > 7) return fail(SONG_ERR_ARG, "learned must be 0..7");
     if (p->stop_interval < 1 || p->stop_min_hops < 0) return fail(SONG_ERR_ARG, "stop_interval >= 1, stop_min_hops >= 0");
     if (o->learned) {
@@ -204,6 +206,7 @@ song::SearchArgs make_args(const song_index* ix, const Plan& p, const void* dq, 
         a.L.min_hops = p.stop_min_hops;
         a.L.interval = p.stop_interval;
         a.L.max_hops = p.stop_max_hops;
+        a.L.confirm = p.stop_confirm;
         a.L.trace = nullptr;
         a.L.trace_count = nullptr;
         a.L.final_dk = nullptr;

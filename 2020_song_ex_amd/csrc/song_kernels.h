@@ -61,6 +61,7 @@ struct LearnedArgs {
     int32_t min_hops;         // first hop at which the stop MLP is consulted
     int32_t interval;         // consult every `interval` hops
     int32_t max_hops;         // stop after this many expansions (0 = no cap)
+    int32_t confirm;          // consecutive consults above the threshold needed to stop (>= 1)
     // training traces (all null outside trace mode)
     float* trace;             // [nq x max_evals x (LM_F + 1)]: features, k-th best distance
     int32_t* trace_count;     // [nq]
@@ -126,7 +127,7 @@ hipError_t launch_learned_forward(const LearnedArgs& L, const PlanArgs& P, int w
 // by predicted cost, most expensive first (counting sort over 256 bins).
 hipError_t launch_plan(const PlanArgs& P, int metric, int32_t* order, hipStream_t st);
 
-// metric: METRIC_*; S: set slots per lane (cap <= 64*S, S in {1,2,3,4,6,8});
+// metric: METRIC_*; S: set slots per lane (cap <= 64*S, S in {1,2,3,4,6,7,8});
 // G: lanes per row (power of two); qch: 16-byte chunks per lane (G*qch*4 >= dstride).
 hipError_t launch_search(const SearchArgs& a, int metric, int S, int G, int qch, const LaunchShape& shape,
                          hipStream_t st);

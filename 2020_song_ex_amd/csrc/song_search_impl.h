@@ -615,6 +615,7 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
     // consulted hops: min_hops, min_hops + interval, ... (hop counts start at
     // 1, so with min_hops 0 the first is interval)
     int next_eval = LEARN ? (P.L.min_hops > 0 ? P.L.min_hops : P.L.interval) : 0;
+    int stop_votes = 0;
     int last_insert_hop = 0;
 
     // Bloom-test a row of ids (lane order), compute the distances of the new
@@ -724,7 +725,9 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
                     n_evals++;
                 } else {
                     const float logit = stop_logit(stop_lds, f, P.L.s_b3, lane, stage_d);
-                    if (logit > P.L.stop_logit) {  // a NaN logit never stops
+                    // a NaN logit never stops; a stop needs `confirm` consecutive votes
+                    stop_votes = logit > P.L.stop_logit ? stop_votes + 1 : 0;
+                    if (stop_votes >= P.L.confirm) {
                         tr_early = 2;
                         break;
                     }
@@ -1002,6 +1005,7 @@ hipError_t launch_s(const SearchArgs& a, int S, int G, int qch, const LaunchShap
     case 3: return launch_g<METRIC, 3>(a, G, qch, wpb, st);
     case 4: return launch_g<METRIC, 4>(a, G, qch, wpb, st);
     case 6: return launch_g<METRIC, 6>(a, G, qch, wpb, st);
+    case 7: return launch_g<METRIC, 7>(a, G, qch, wpb, st);
     case 8: return launch_g<METRIC, 8>(a, G, qch, wpb, st);
     }
     return hipErrorInvalidValue;

@@ -210,7 +210,8 @@ def main():
     log(f"plain: queue {ef} {plain_params}, recall@10 {plain_rec:.4f}, {plain_qps:.0f} queries/s")
 
     def slots_for(q):
-        return 1 if q <= 64 else 2 if q <= 128 else 3 if q <= 192 else 4 if q <= 256 else 6 if q <= 384 else 8
+        return (1 if q <= 64 else 2 if q <= 128 else 3 if q <= 192 else 4 if q <= 256 else 6 if q <= 384
+                else 7 if q <= 448 else 8)
 
     def roofline_of(trace_rows, kernel_ms, kname, key):
         """Algorithmic bytes of one launch / its HIP-event duration; HBM traffic per launch
@@ -261,9 +262,9 @@ def main():
             log(f"learned model trained in {time.time() - t0:.1f}s")
             if args.model:
                 model.save(args.model)
-        grid = {"ef": [352, 384, 448], "learned": [2, 6], "stop_threshold": [2.5, 3.0, 3.5, 4.0],
-                "stop_max_hops": [0, 500, 600, 700], "stop_min_hops": [16], "stop_interval": [4], "mode": [1],
-                "waves_per_block": [8]}
+        grid = {"ef": [384, 448, 512], "learned": [2], "stop_threshold": [2.0, 2.5, 3.0, 3.5],
+                "stop_confirm": [1, 2, 3], "stop_max_hops": [0, 600], "stop_min_hops": [16], "stop_interval": [4],
+                "mode": [1], "waves_per_block": [8]}
         if args.learned_config:
             grid = {kk: [v] for kk, v in json.loads(args.learned_config).items()}
         res = autotune.tune(idx, valid, vgt, args.k, grid, tune_target, log=log)

@@ -129,7 +129,9 @@ typedef struct song_search_opts {
     int32_t stop_interval;   /* consult every this many hops (>= 1) */
     int32_t stop_max_hops;   /* with learned bit 1: also stop once a query has
                                 expanded this many nodes (0 = no cap) */
-    int32_t reserved[4];
+    int32_t stop_confirm;    /* stop after this many consecutive consults above
+                                the threshold (0 = 1) */
+    int32_t reserved[3];
 } song_search_opts;
 
 enum { SONG_MODE_PLAIN = 0, SONG_MODE_PERSISTENT = 1 };

@@ -185,6 +185,19 @@ def test_hop_cap_matches_oracle(song, oracle, setup, cap):
     assert (r.trace[:, 11] == 3).any()
 
 
+def test_confirmed_stop_runs_longer(song, setup):
+    """Needing two consecutive stop votes never stops earlier than one vote,
+    and one confirmation is the default."""
+    base, q, adj, idx, model = setup
+    kw = dict(learned=2, stop_threshold=1.0, stop_min_hops=8, stop_interval=2)
+    one = idx.search(q, 10, 128, **kw)
+    dflt = idx.search(q, 10, 128, stop_confirm=1, **kw)
+    two = idx.search(q, 10, 128, stop_confirm=2, **kw)
+    np.testing.assert_array_equal(one.trace, dflt.trace)
+    assert (two.trace[:, 2] >= one.trace[:, 2]).all()
+    assert two.trace[:, 2].mean() > one.trace[:, 2].mean()
+
+
 def test_trace_labels(song, setup):
     base, q, adj, idx, model = setup
     N = song._native
@@ -225,3 +238,36 @@ def test_learned_search_saves_work(song, setup):
     assert lr.trace[:, 2].mean() < plain.trace[:, 2].mean()
     assert rec(lr.ids) > rec(plain.ids) - 0.05, (rec(lr.ids), rec(plain.ids))
     assert (lr.trace[:, 11] == 2).any()
+
+
+@pytest.mark.parametrize("metric", ["hamming", "ip"])
+def test_learned_search_other_metrics(song, datasets, metric):
+    """The networks train and run on Hamming codes (256-bit, +-1 projection) and on
+    inner-product data; the stop saves work at a small recall cost and never
+    firing returns the plain search."""
+    vec, qv = datasets.vmf_mix(12000, 1600, dim=64, n_centres=40, kappa=20.0, seed=4)
+    if metric == "hamming":
+        base, planes = datasets.hyperplane_codes(vec, seed=5)
+        qs, _ = datasets.hyperplane_codes(qv, planes=planes)
+    else:
+        base, qs = vec, qv
+    adj = song.nsw_build(base, metric=metric, block=32, exact_order=False)
+    idx = song.Index(base, adj, metric=metric)
+    try:
+        q, train = qs[:300], qs[300:]
+        model = learned.fit(idx, base, train, k=10, ef=128, entry_epochs=5, stop_epochs=3, cost_epochs=5,
+                            stop_min_hops=8, stop_interval=2)
+        gt, _, _ = idx.bruteforce(q, 10)
+
+        def rec(ids):
+            return np.mean([len(set(a) & set(b)) / 10 for a, b in zip(ids, gt)])
+
+        plain = idx.search(q, 10, 128)
+        never = idx.search(q, 10, 128, learned=2, stop_threshold=1e30, stop_min_hops=1)
+        np.testing.assert_array_equal(never.ids, plain.ids)
+        np.testing.assert_array_equal(never.trace, plain.trace)
+        lr = idx.search(q, 10, 128, learned=6, stop_threshold=2.0, stop_min_hops=8, stop_interval=2, mode=1)
+        assert lr.trace[:, 2].mean() < plain.trace[:, 2].mean()
+        assert rec(lr.ids) > rec(plain.ids) - 0.08, (rec(lr.ids), rec(plain.ids))
+    finally:
+        idx.close()

@@ -60,7 +60,7 @@ def test_config1_exact(song, oracle, c1):
 
 
 @pytest.mark.parametrize("k,ef", [(1, 1), (10, 10), (10, 32), (10, 50), (10, 100), (10, 150), (10, 200),
-                                  (100, 100), (100, 256), (10, 300), (100, 384), (10, 512), (64, 64), (65, 65)])
+                                  (100, 100), (100, 256), (10, 300), (100, 384), (10, 448), (10, 512), (64, 64), (65, 65)])
 def test_queue_sizes(song, oracle, c1, k, ef):
     cfg, adj = c1
     r, o = run_both(song, oracle, cfg["base"], adj, cfg["queries"][:60], k, ef)
@@ -94,6 +94,15 @@ def test_dims(song, oracle, dim):
     adj = song.nsw_build(base, block=32, exact_order=False)
     r, o = run_both(song, oracle, base, adj, q, 10, 40)
     assert_same(r, o, f"dim={dim}")
+
+
+def test_beta_mix_960(song, oracle, datasets):
+    """960-d Beta-perturbed mixture (float data, config-3 shape): exact through the
+    fixed summation order, queue 50 as the reference's default."""
+    base, q = datasets.beta_mix(3000, 30, dim=960, n_centres=30, seed=3)
+    adj = song.nsw_build(base, block=32, exact_order=False)
+    r, o = run_both(song, oracle, base, adj, q, 10, 50)
+    assert_same(r, o, "beta960")
 
 
 def test_inner_product(song, oracle, datasets):
