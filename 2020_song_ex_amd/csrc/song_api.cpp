@@ -101,7 +101,8 @@ size_t host_row_bytes(const song_index* ix) {
 
 struct Plan {
     int ef, S, G, qch, bloom_words, bloom_shift, n_seeds, batch, persistent;
-    int learned;  // bit 0 entry, bit 1 stop, bit 2 cost-ordered queue
+    int learned;  // bit 0 entry, bit 1 stop, bit 2 cost-ordered queue, bit 3 keep the reference seeds
+    int n_entry;  // seed-row length: n_seeds, or 2 * n_seeds with bit 3
     float stop_threshold;
     int stop_min_hops, stop_interval, stop_max_hops, stop_confirm;
     song::LaunchShape shape;
@@ -162,7 +163,10 @@ int make_plan(const song_index* ix, int32_t nq, int32_t k, int32_t ef, const son
     if (o->stop_max_hops < 0) return fail(SONG_ERR_ARG, "stop_max_hops must be >= 0");
     p->stop_confirm = o->stop_confirm ? o->stop_confirm : 1;
     if (o->stop_confirm < 0 || o->stop_confirm > 64) return fail(SONG_ERR_ARG, "stop_confirm must be in [0, 64]");
-    if (o->learned < 0 || o->learned > 7) return fail(SONG_ERR_ARG, "learned must be 0..7");
+    if (o->learned < 0 || o->learned > 15) return fail(SONG_ERR_ARG, "learned must be 0..15");
+    if ((o->learned & 8) && !(o->learned & 1)) return fail(SONG_ERR_ARG, "learned bit 3 needs the entry network (bit 0)");
+    p->n_entry = (o->learned & 8) ? 2 * p->n_seeds : p->n_seeds;
+    if (p->n_entry > 64) return fail(SONG_ERR_ARG, "with learned bit 3, n_seeds must be <= 32");
     if (p->stop_interval < 1 || p->stop_min_hops < 0) return fail(SONG_ERR_ARG, "stop_interval >= 1, stop_min_hops >= 0");
     if (o->learned) {
         if (!ix->has_model) return fail(SONG_ERR_STATE, "learned search needs a model (song_index_set_model)");
@@ -191,7 +195,7 @@ song::SearchArgs make_args(const song_index* ix, const Plan& p, const void* dq, 
     a.nq = nq;
     a.bloom_words = p.bloom_words;
     a.bloom_shift = p.bloom_shift;
-    a.n_entry = p.n_seeds;
+    a.n_entry = p.n_entry;
     a.ks = nullptr;
     a.efs = nullptr;
     a.k_stride = k;
@@ -281,10 +285,11 @@ hipError_t run_plan(const song_index* ix, const Plan& p, song::SearchArgs& a, in
     song::PlanArgs P = ix->plan;
     P.queries = a.queries;
     P.nq = a.nq;
-    P.n_entry = p.n_seeds;
+    P.n_entry = p.n_entry;
+    P.n_ref = (p.learned & 8) ? p.n_seeds : 0;
     P.entries = (p.learned & 1) ? ws : nullptr;
-    P.cost = (p.learned & 4) ? (float*)(ws + (size_t)a.nq * p.n_seeds) : nullptr;
-    int32_t* order = (p.learned & 4) ? ws + (size_t)a.nq * (p.n_seeds + 1) : nullptr;
+    P.cost = (p.learned & 4) ? (float*)(ws + (size_t)a.nq * p.n_entry) : nullptr;
+    int32_t* order = (p.learned & 4) ? ws + (size_t)a.nq * (p.n_entry + 1) : nullptr;
     hipError_t e = song::launch_plan(P, ix->metric, order, st);
     a.entries = P.entries;
     a.order = order;
@@ -544,7 +549,7 @@ int song_search(song_index* ix, const void* queries, int32_t nq, int32_t k, int3
     const int64_t bmax = std::min<int64_t>(p.batch, std::max<int32_t>(nq, 1));
     rc = ensure_ws(ix, bmax, k, dists_out != nullptr, trace_out != nullptr, &tm.alloc_ms);
     if (rc) return rc;
-    if (needs_plan(p) && (rc = ensure_plan_ws(ix, bmax, p.n_seeds))) return rc;
+    if (needs_plan(p) && (rc = ensure_plan_ws(ix, bmax, p.n_entry))) return rc;
     const size_t hb = host_row_bytes(ix), rb = row_bytes(ix);
     auto t0 = std::chrono::steady_clock::now();
     int bi = 0;
@@ -609,7 +614,7 @@ int song_search_device(song_index* ix, const void* d_queries, int32_t nq, int32_
     std::memset(&tm, 0, sizeof tm);
     tm.ef_used = p.ef;
     const size_t rb = row_bytes(ix);
-    if (needs_plan(p) && (rc = ensure_plan_ws(ix, std::min<int64_t>(p.batch, std::max<int32_t>(nq, 1)), p.n_seeds)))
+    if (needs_plan(p) && (rc = ensure_plan_ws(ix, std::min<int64_t>(p.batch, std::max<int32_t>(nq, 1)), p.n_entry)))
         return rc;
     auto t0 = std::chrono::steady_clock::now();
     HIP_TRY(hipEventRecord(ix->ev[1], ix->stream), "hipEventRecord");
@@ -646,7 +651,7 @@ int song_search_device_mixed(song_index* ix, const void* d_queries, int32_t nq, 
     song_timing tm;
     std::memset(&tm, 0, sizeof tm);
     tm.ef_used = p.ef;
-    if (needs_plan(p) && (rc = ensure_plan_ws(ix, std::max<int32_t>(nq, 1), p.n_seeds))) return rc;
+    if (needs_plan(p) && (rc = ensure_plan_ws(ix, std::max<int32_t>(nq, 1), p.n_entry))) return rc;
     auto t0 = std::chrono::steady_clock::now();
     HIP_TRY(hipEventRecord(ix->ev[1], ix->stream), "hipEventRecord");
     if (nq > 0) {
@@ -760,7 +765,7 @@ int song_learn_trace(song_index* ix, const void* d_queries, int32_t nq, int32_t 
     if (rc) return rc;
     std::lock_guard<std::mutex> lock(ix->mu);
     HIP_TRY(hipSetDevice(ix->device), "hipSetDevice");
-    if (needs_plan(p) && (rc = ensure_plan_ws(ix, std::max<int32_t>(nq, 1), p.n_seeds))) return rc;
+    if (needs_plan(p) && (rc = ensure_plan_ws(ix, std::max<int32_t>(nq, 1), p.n_entry))) return rc;
     int32_t* d_ids = nullptr;
     HIP_TRY(hipMalloc(&d_ids, (size_t)std::max(nq, 1) * k * 4), "hipMalloc ids");
     hipError_t e = hipSuccess;

@@ -45,7 +45,8 @@ struct PlanArgs {
     const float* e_b3;        // [LM_C]
     const int32_t* pivots;    // [LM_C] node ids (< n, checked on the host)
     const float* c_weights;   // LM_COST_FLOATS (16 -> 64 -> 64 -> 1), or null
-    int32_t n_entry;          // seeds per query
+    int32_t n_entry;          // seeds per query (row length of entries)
+    int32_t n_ref;            // leading reference seeds (p * n / n_ref) kept in each row, 0 = none
     int32_t* entries;         // [nq x n_entry] out, or null (no entry network)
     float* cost;              // [nq] out (predicted standardised log2 hops), or null
 };

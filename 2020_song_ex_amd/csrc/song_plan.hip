@@ -99,12 +99,17 @@ __global__ void __launch_bounds__(256) plan_kernel(PlanArgs P) {
     if (P.entries) {
         float score = entry_scores(P, z, lane);
         int sid = 0;
+        // the row: n_ref reference seeds, then the best-scoring pivots
+        const int nr = P.n_ref;
         if (__ballot(!(score == score))) {
             // a score that is not a number: the reference's evenly spaced seeds
-            sid = (int)(((uint64_t)lane * (uint64_t)P.n) / (uint64_t)P.n_entry);
+            // (repeated to fill the row; repeats are visited-set hits)
+            const int m = nr > 0 ? nr : P.n_entry;
+            sid = (int)(((uint64_t)(lane % m) * (uint64_t)P.n) / (uint64_t)m);
         } else {
+            if (lane < nr) sid = (int)(((uint64_t)lane * (uint64_t)P.n) / (uint64_t)nr);
             const int piv = P.pivots[lane];
-            for (int r = 0; r < P.n_entry; r++) {
+            for (int r = nr; r < P.n_entry; r++) {
                 const float m = wave_max(score);
                 const int l = ffs64(__ballot(score == m));
                 const int id = __builtin_amdgcn_readlane(piv, l);

@@ -120,7 +120,9 @@ typedef struct song_search_opts {
                                 logit exceeds stop_threshold; bit 2: run the
                                 queries in order of the cost network's
                                 prediction, most expensive first (results are
-                                unchanged, only when each query starts).
+                                unchanged, only when each query starts);
+                                bit 3 (with bit 0): keep the n_seeds reference
+                                seeds and add n_seeds learned ones.
                                 Needs a model (song_index_set_model; bit 2 a
                                 model with a cost network), k <= 64,
                                 dim <= 1024. */

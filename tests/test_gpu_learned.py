@@ -80,11 +80,13 @@ def test_nan_model_falls_back_to_plain(song, setup):
     bad.set_on(idx)
     try:
         r = idx.search(q, 10, 128, learned=3, stop_threshold=0.0, stop_min_hops=1)
+        r2 = idx.search(q, 10, 128, learned=11, stop_threshold=0.0, stop_min_hops=1)
     finally:
         model.set_on(idx)
-    np.testing.assert_array_equal(r.ids, plain.ids)
-    np.testing.assert_array_equal(r.dists, plain.dists)
-    np.testing.assert_array_equal(r.trace, plain.trace)
+    for x in (r, r2):
+        np.testing.assert_array_equal(x.ids, plain.ids)
+        np.testing.assert_array_equal(x.dists, plain.dists)
+        np.testing.assert_array_equal(x.trace, plain.trace)
 
 
 def test_stop_never_fires_equals_plain(song, setup):
@@ -196,6 +198,21 @@ def test_confirmed_stop_runs_longer(song, setup):
     np.testing.assert_array_equal(one.trace, dflt.trace)
     assert (two.trace[:, 2] >= one.trace[:, 2]).all()
     assert two.trace[:, 2].mean() > one.trace[:, 2].mean()
+
+
+def test_augmented_entry_matches_oracle_with_those_seeds(song, oracle, setup):
+    """Learned bit 3: each seed row is the 4 reference seeds followed by the 4
+    best pivots; the search equals the oracle run from that row."""
+    base, q, adj, idx, model = setup
+    r = idx.search(q, 10, 128, learned=9)
+    scores = learned.mlp_forward(model.entry_w, model.project(q, 0).astype(np.float32), 16, 64)
+    order = np.argsort(-scores, axis=1, kind="stable")[:, :4]
+    ref = np.array([(p * base.shape[0]) // 4 for p in range(4)], np.int32)
+    seeds = np.concatenate([np.tile(ref, (q.shape[0], 1)), model.pivots[order]], axis=1)
+    oi, od, ot = oracle.search(base, adj, q, 10, 128, block=32, bloom_bits=1 << 14, group=4, entries=seeds)
+    assert (r.ids == oi).all(1).mean() > 0.97
+    with pytest.raises(song.SongError, match="bit 3"):
+        idx.search(q[:2], 10, 128, learned=8)
 
 
 def test_trace_labels(song, setup):
