@@ -145,10 +145,11 @@ int make_plan(const song_index* ix, int32_t nq, int32_t k, int32_t ef, const son
         if (G) {
             if (!is_pow2(G) || G < 2 || G > 64) return fail(SONG_ERR_ARG, "lanes_per_row must be a power of two in [2, 64]");
         } else {
+            // two 16-byte chunks per lane (fewest registers per row in flight)
             G = 2;
-            while (G < 64 && G * 4 < d4) G *= 2;
+            while (G < 64 && G * 2 < d4) G *= 2;
         }
-        int qch = 4;
+        int qch = 2;
         while (G * qch < d4) qch *= 2;
         if (qch > 4 && G != 64) return fail(SONG_ERR_ARG, "lanes_per_row too small for this dim");
         if (qch > 16) return fail(SONG_ERR_ARG, "dim > 4096 is not supported");

@@ -270,23 +270,76 @@ __device__ __forceinline__ uint64_t bloom_row_pairs(unsigned long long* bloom, u
 
 // ---------------------------------------------------------------- the two sets
 
-// Unsorted bounded set of (dist, id): the distance of entry i in VGPR slot
-// i/64 of lane i%64, its id in LDS at ids[i].  Every scan (min, max, first
-// index of a value) runs on the distances; ids are only read or written at one
-// known index, so they cost one LDS access instead of a select per slot.
-// Slots at or beyond sz hold NaN, which fminf / fmaxf ignore and == never
-// matches, so scans need no validity test.  The maximum is cached (wave
-// uniform) and recomputed only after it can have changed.
-template <int S>
+// Set entries are compared as 32-bit integer keys, not floats: integer
+// min / max fuse with DPP and need no NaN canonicalisation, which halves the
+// cost of a wave reduction.  The key of a distance d is its bit pattern,
+// mapped so that signed integer order is float order:
+//  * L2 and Hamming distances are >= +0 (never -0, never negative), whose bit
+//    patterns already order as integers -- the key is the bit pattern;
+//  * inner-product distances (-dot) can be negative: negative patterns get
+//    their magnitude bits flipped (an involution, so the same map decodes).
+// The empty slot holds INT_MIN: below every key for a signed max, and (as
+// 2^31) above every non-negative key for an unsigned min.
+constexpr int KEY_EMPTY = (int)0x80000000u;
+
+template <bool SIGNED>
+__device__ __forceinline__ int dist_key(float d) {
+    const int b = __float_as_int(d);
+    if constexpr (SIGNED) return b ^ ((b >> 31) & 0x7fffffff);
+    else return b;
+}
+template <bool SIGNED>
+__device__ __forceinline__ float key_dist(int k) {
+    if constexpr (SIGNED) return __int_as_float(k ^ ((k >> 31) & 0x7fffffff));
+    else return __int_as_float(k);
+}
+
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+// In-row steps as in wave_max, then row_bcast:15 (rows 1 and 3 take lane 15
+// of the row below) and row_bcast:31 (rows 2 and 3 take lane 31): lane 63
+// ends with the maximum of all four rows.  Rows a broadcast skips keep the
+// identity, so the max leaves them as they were.
+constexpr int DPP_ROW_BCAST15 = 0x142;
+constexpr int DPP_ROW_BCAST31 = 0x143;
+__device__ __forceinline__ int wave_max_i(int v) {
+    v = max(v, dpp_i<DPP_XOR1>(v));
+    v = max(v, dpp_i<DPP_XOR2>(v));
+    v = max(v, dpp_i<DPP_HALF_MIRROR>(v));
+    v = max(v, dpp_i<DPP_MIRROR>(v));
+    v = max(v, __builtin_amdgcn_update_dpp((int)0x80000000u, v, DPP_ROW_BCAST15, 0xA, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp((int)0x80000000u, v, DPP_ROW_BCAST31, 0xC, 0xF, false));
+    return __builtin_amdgcn_readlane(v, 63);
+}
+__device__ __forceinline__ uint32_t wave_min_u(uint32_t v) {
+    v = min(v, (uint32_t)dpp_i<DPP_XOR1>((int)v));
+    v = min(v, (uint32_t)dpp_i<DPP_XOR2>((int)v));
+    v = min(v, (uint32_t)dpp_i<DPP_HALF_MIRROR>((int)v));
+    v = min(v, (uint32_t)dpp_i<DPP_MIRROR>((int)v));
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, DPP_ROW_BCAST15, 0xA, 0xF, false));
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, DPP_ROW_BCAST31, 0xC, 0xF, false));
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+// Unsorted bounded set of (dist, id): the key of entry i in VGPR slot i/64 of
+// lane i%64, its id in LDS at ids[i].  Every scan (min, max, first index of a
+// key) runs on the keys; ids are only read or written at one known index, so
+// they cost one LDS access instead of a select per slot.  Slots at or beyond
+// sz hold KEY_EMPTY, which no scan picks and == never matches, so scans need
+// no validity test.  The maximum is cached (wave uniform) and recomputed only
+// after it can have changed.  SIGNED: keys of inner-product distances.
+template <int S, bool SIGNED>
 struct WSet {
     // ext_vector_type keeps the slots in VGPRs: a plain array gets its
     // per-slot selects folded back into an indexed scratch access.
-    typedef float fvec __attribute__((ext_vector_type(S == 1 ? 2 : S)));
-    fvec d;
+    typedef int kvec __attribute__((ext_vector_type(S == 1 ? 2 : S)));
+    kvec d;
     int* ids;  // this wave's LDS array of 64 * S ids
     int sz;    // wave-uniform
     int cap;   // wave-uniform
-    float mx;  // cached maximum, valid when mx_ok
+    int mx;    // cached maximum key, valid when mx_ok
     bool mx_ok;
 
     __device__ __forceinline__ void init(int c, int* lds_ids) {
@@ -294,100 +347,109 @@ struct WSet {
         sz = 0;
         cap = c < 1 ? 1 : c;
         mx_ok = false;
-        mx = 0.f;
+        mx = KEY_EMPTY;
 #pragma unroll
-        for (int s = 0; s < S; s++) d[s] = __builtin_nanf("");
+        for (int s = 0; s < S; s++) d[s] = KEY_EMPTY;
     }
-    __device__ __forceinline__ float vmax(int lane) const {
-        float m = -INFINITY;
+    __device__ __forceinline__ int vmax(int lane) const {
+        int m = d[0];
 #pragma unroll
-        for (int s = 0; s < S; s++) m = fmaxf(m, d[s]);
-        return wave_max(m);
+        for (int s = 1; s < S; s++) m = max(m, d[s]);
+        return wave_max_i(m);
     }
-    __device__ __forceinline__ float cmax(int lane) {
+    __device__ __forceinline__ int cmax(int lane) {
         if (!mx_ok) {
             mx = vmax(lane);
             mx_ok = true;
         }
         return mx;
     }
-    __device__ __forceinline__ float vmin(int lane) const {
-        float m = INFINITY;
-#pragma unroll
-        for (int s = 0; s < S; s++) m = fminf(m, d[s]);
-        return wave_min(m);
-    }
-    // First index whose value equals v (v is a value present in the set).
-    __device__ __forceinline__ int first_eq(float v, int lane) const {
+    // Minimum key of the slots k (the set's own or a copy of them).
+    __device__ __forceinline__ static int kmin(const kvec& k) {
+        uint32_t m = 0xffffffffu;
 #pragma unroll
         for (int s = 0; s < S; s++) {
-            uint64_t b = __ballot(d[s] == v);
+            uint32_t u = (uint32_t)k[s];
+            // signed keys: move to unsigned order, the empty slot to the top
+            if constexpr (SIGNED) u = k[s] == KEY_EMPTY ? 0xffffffffu : u ^ 0x80000000u;
+            m = min(m, u);
+        }
+        m = wave_min_u(m);
+        return SIGNED ? (int)(m ^ 0x80000000u) : (int)m;
+    }
+    __device__ __forceinline__ int vmin(int lane) const { return kmin(d); }
+    // First index whose key equals v (v is a key present in the set).
+    __device__ __forceinline__ static int first_eq_of(const kvec& k, int v) {
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+            uint64_t b = __ballot(k[s] == v);
             if (b) return s * 64 + ffs64(b);
         }
         return 0;
     }
-    __device__ __forceinline__ float get_d(int idx) const {
+    __device__ __forceinline__ int first_eq(int v, int lane) const { return first_eq_of(d, v); }
+    __device__ __forceinline__ int get_d(int idx) const {
         const int slot = idx >> 6, ln = idx & 63;
-        float dd = d[0];
+        int dd = d[0];
 #pragma unroll
         for (int s = 1; s < S; s++)
             if (slot == s) dd = d[s];
-        return readlane_f(dd, ln);
+        return __builtin_amdgcn_readlane(dd, ln);
     }
-    __device__ __forceinline__ void get(int idx, float& dv, int& iv) const {
-        dv = get_d(idx);
+    __device__ __forceinline__ void get(int idx, int& kv, int& iv) const {
+        kv = get_d(idx);
         iv = ids[idx];
     }
-    // Branch-free on the distances: every lane selects, only lane idx%64 in
-    // slot idx/64 changes; one lane writes the id.
-    __device__ __forceinline__ void put_d(int idx, float dv, int lane) {
+    // Branch-free on the keys: every lane selects, only lane idx%64 in slot
+    // idx/64 changes; one lane writes the id.
+    __device__ __forceinline__ void put_d(int idx, int kv, int lane) {
         const int slot = idx >> 6;
         const bool mine = lane == (idx & 63);
 #pragma unroll
-        for (int s = 0; s < S; s++) d[s] = (mine && slot == s) ? dv : d[s];
+        for (int s = 0; s < S; s++) d[s] = (mine && slot == s) ? kv : d[s];
     }
-    __device__ __forceinline__ void put(int idx, float dv, int iv, int lane) {
-        put_d(idx, dv, lane);
+    __device__ __forceinline__ void put(int idx, int kv, int iv, int lane) {
+        put_d(idx, kv, lane);
         if (lane == 0) ids[idx] = iv;
     }
     // Remove entry p, filling the hole with the last entry (gpuHeap.h:44-48);
-    // the vacated last slot becomes NaN.  The caller keeps mx_ok as fits.
+    // the vacated last slot becomes empty.  The caller keeps mx_ok as fits.
     __device__ __forceinline__ void remove_at(int p, int lane) {
         const int last = sz - 1;
-        const float ld = get_d(last);
+        const int lk = get_d(last);
         const int li = ids[last];
-        put(p, ld, li, lane);
-        put_d(last, __builtin_nanf(""), lane);
+        put(p, lk, li, lane);
+        put_d(last, KEY_EMPTY, lane);
         sz--;
     }
-    // Push items it_d/it_id held by lanes 0..F-1, in lane order, with the
+    // Push items it_k/it_id held by lanes 0..F-1, in lane order, with the
     // Candidates::push / Results::try_push rule (gpuHeap.h:23-36, 78-87).
     // Returns how many were accepted.
-    __device__ __forceinline__ int push_items(float it_d, int it_id, int F, int lane) {
+    __device__ __forceinline__ int push_items(int it_k, int it_id, int F, int lane) {
         int accepted = 0;
         const int a = min(F, cap - sz);
         if (a > 0) {
             const int src = (lane - sz) & 63;
-            const float md = __shfl(it_d, src, 64);
+            const int mk = __shfl(it_k, src, 64);
             const int slot = src < a ? (sz + src) >> 6 : -1;
 #pragma unroll
-            for (int s = 0; s < S; s++) d[s] = slot == s ? md : d[s];
+            for (int s = 0; s < S; s++) d[s] = slot == s ? mk : d[s];
             if (lane < a) ids[sz + lane] = it_id;
             sz += a;
             accepted = a;
             mx_ok = false;
         }
         if (F > a) {
-            float M = cmax(lane);
+            int M = cmax(lane);
             int start = a;
             for (;;) {
-                const uint64_t m = __ballot(lane >= start && lane < F && it_d < M);
+                const uint64_t m = __ballot(lane >= start && lane < F && it_k < M);
                 if (m == 0ull) break;
                 const int r = ffs64(m);
-                const float dv = readlane_f(it_d, r);
+                const int kv = __builtin_amdgcn_readlane(it_k, r);
                 const int iv = __builtin_amdgcn_readlane(it_id, r);
                 const int p = first_eq(M, lane);
-                put(p, dv, iv, lane);
+                put(p, kv, iv, lane);
                 accepted++;
                 mx_ok = false;
                 start = r + 1;
@@ -404,59 +466,71 @@ struct WSet {
 // Distances of the rows stage_id[0..F) to the query; lane 0 of each group of
 // G lanes writes stage_d[r].  Lane t of a group accumulates the 16-byte
 // chunks c = t, t+G, ... (c < dstride/4) of the row in order, with explicit
-// fmaf; the group sum is the xor butterfly of group_sum.
+// fmaf; the group sum is the xor butterfly of group_sum.  distance_passes
+// handles passes p0 .. p0+NP-1 (64/G rows each) with all their loads issued
+// before the first sum.
+template <int METRIC, int G, int QCH, int NP>
+__device__ __forceinline__ void distance_passes(const float4* __restrict__ data, int d4, const float4 (&q)[QCH],
+                                                int F, int p0, const int* stage_id, float* stage_d, int lane) {
+    constexpr int NG = 64 / G;
+    const int grp = lane / G, t = lane % G;
+    float4 x[NP][QCH];
+    int rr[NP];
+#pragma unroll
+    for (int pp = 0; pp < NP; pp++) {
+        const int r = (p0 + pp) * NG + grp;
+        rr[pp] = r;
+        const int node = r < F ? stage_id[r] : -1;
+        const float4* row = data + (size_t)(node < 0 ? 0 : node) * d4;
+#pragma unroll
+        for (int i = 0; i < QCH; i++) {
+            const int c = t + i * G;
+            if (node >= 0 && c < d4) x[pp][i] = row[c];
+            else x[pp][i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+    }
+#pragma unroll
+    for (int pp = 0; pp < NP; pp++) {
+        float acc = 0.f;
+#pragma unroll
+        for (int i = 0; i < QCH; i++) {
+            const int c = t + i * G;
+            if (c < d4) {
+                if constexpr (METRIC == METRIC_IP) {
+                    acc = __builtin_fmaf(q[i].x, x[pp][i].x, acc);
+                    acc = __builtin_fmaf(q[i].y, x[pp][i].y, acc);
+                    acc = __builtin_fmaf(q[i].z, x[pp][i].z, acc);
+                    acc = __builtin_fmaf(q[i].w, x[pp][i].w, acc);
+                } else {
+                    float e;
+                    e = q[i].x - x[pp][i].x;
+                    acc = __builtin_fmaf(e, e, acc);
+                    e = q[i].y - x[pp][i].y;
+                    acc = __builtin_fmaf(e, e, acc);
+                    e = q[i].z - x[pp][i].z;
+                    acc = __builtin_fmaf(e, e, acc);
+                    e = q[i].w - x[pp][i].w;
+                    acc = __builtin_fmaf(e, e, acc);
+                }
+            }
+        }
+        acc = group_sum<G>(acc);
+        if (t == 0 && rr[pp] < F) stage_d[rr[pp]] = METRIC == METRIC_IP ? -acc : acc;
+    }
+}
+
+// NP passes at a time while that many remain, then one at a time (a row's sum
+// does not depend on which passes share a step).
 template <int METRIC, int G, int QCH, int NP>
 __device__ __forceinline__ void float_distances(const float4* __restrict__ data, int d4,
                                                 const float4 (&q)[QCH], int F,
                                                 const int* stage_id, float* stage_d, int lane) {
     constexpr int NG = 64 / G;
-    const int grp = lane / G, t = lane % G;
     const int passes = (F + NG - 1) / NG;
-    for (int p0 = 0; p0 < passes; p0 += NP) {
-        float4 x[NP][QCH];
-        int rr[NP];
-#pragma unroll
-        for (int pp = 0; pp < NP; pp++) {
-            const int r = (p0 + pp) * NG + grp;
-            rr[pp] = r;
-            const int node = r < F ? stage_id[r] : -1;
-            const float4* row = data + (size_t)(node < 0 ? 0 : node) * d4;
-#pragma unroll
-            for (int i = 0; i < QCH; i++) {
-                const int c = t + i * G;
-                if (node >= 0 && c < d4) x[pp][i] = row[c];
-                else x[pp][i] = make_float4(0.f, 0.f, 0.f, 0.f);
-            }
-        }
-#pragma unroll
-        for (int pp = 0; pp < NP; pp++) {
-            float acc = 0.f;
-#pragma unroll
-            for (int i = 0; i < QCH; i++) {
-                const int c = t + i * G;
-                if (c < d4) {
-                    if constexpr (METRIC == METRIC_IP) {
-                        acc = __builtin_fmaf(q[i].x, x[pp][i].x, acc);
-                        acc = __builtin_fmaf(q[i].y, x[pp][i].y, acc);
-                        acc = __builtin_fmaf(q[i].z, x[pp][i].z, acc);
-                        acc = __builtin_fmaf(q[i].w, x[pp][i].w, acc);
-                    } else {
-                        float e;
-                        e = q[i].x - x[pp][i].x;
-                        acc = __builtin_fmaf(e, e, acc);
-                        e = q[i].y - x[pp][i].y;
-                        acc = __builtin_fmaf(e, e, acc);
-                        e = q[i].z - x[pp][i].z;
-                        acc = __builtin_fmaf(e, e, acc);
-                        e = q[i].w - x[pp][i].w;
-                        acc = __builtin_fmaf(e, e, acc);
-                    }
-                }
-            }
-            acc = group_sum<G>(acc);
-            if (t == 0 && rr[pp] < F) stage_d[rr[pp]] = METRIC == METRIC_IP ? -acc : acc;
-        }
-    }
+    int p0 = 0;
+    for (; p0 + NP <= passes; p0 += NP) distance_passes<METRIC, G, QCH, NP>(data, d4, q, F, p0, stage_id, stage_d, lane);
+    if constexpr (NP > 1)
+        for (; p0 < passes; p0++) distance_passes<METRIC, G, QCH, 1>(data, d4, q, F, p0, stage_id, stage_d, lane);
 }
 
 // Hamming on 256-bit codes (4 x u64): two lanes per code, 16 bytes each
@@ -595,7 +669,8 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
     }
     wave_lds_sync();
 
-    WSet<S> cand, res;
+    constexpr bool SK = METRIC == METRIC_IP;
+    WSet<S, SK> cand, res;
     cand.init(ef, set_ids);
     res.init(ef, set_ids + 64 * S);
     // QueryTrace, telemetry.h:5-18 (wave-uniform counters)
@@ -618,9 +693,12 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
     int stop_votes = 0;
     int last_insert_hop = 0;
 
-    // Bloom-test a row of ids (lane order), compute the distances of the new
-    // ones and push them to both sets in order.  Returns the number of new ids.
-    auto visit_row = [&](int id, bool valid, int width) __attribute__((always_inline)) -> int {
+    // Each expansion is gather (Bloom test-then-set of a row of ids in lane
+    // order, distances of the fresh ones: lane r < F holds the r-th fresh id
+    // and its key), then the pushes of those F items to both sets in order.
+    int it_k = 0, it_id = -1;
+    float it_d = INFINITY;
+    auto gather_row = [&](int id, bool valid, int width) __attribute__((always_inline)) -> int {
         const uint64_t acc = width <= 32 ? bloom_row_pairs(bloom, wmask, wshift, nbits_mask, id, valid, lane)
                                          : bloom_row(bloom, wmask, wshift, nbits_mask, id, valid);
         const int F = __popcll(acc);
@@ -635,18 +713,51 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
             float_distances<METRIC, G, QCH, 2>((const float4*)P.data, d4, qf, F, stage_id, stage_d, lane);
         }
         wave_lds_sync();
-        const float it_d = lane < F ? stage_d[lane] : INFINITY;
-        const int it_id = lane < F ? stage_id[lane] : -1;
-        tr_fpush += cand.push_items(it_d, it_id, F, lane);
-        tr_rupd += res.push_items(it_d, it_id, F, lane);
-        tr_peakf = max(tr_peakf, cand.sz);
-        if constexpr (LEARN) {
-            const int before = tk.inserts;
-            tk.insert(it_d, F, lane);
-            if (tk.inserts != before) last_insert_hop = tr_expanded;
+        it_d = lane < F ? stage_d[lane] : INFINITY;
+        it_id = lane < F ? stage_id[lane] : -1;
+        it_k = dist_key<SK>(it_d);
+        return F;
+    };
+    auto push_row = [&](int F) __attribute__((always_inline)) {
+        if (F > 0) {
+            tr_fpush += cand.push_items(it_k, it_id, F, lane);
+            tr_rupd += res.push_items(it_k, it_id, F, lane);
+            tr_peakf = max(tr_peakf, cand.sz);
+            if constexpr (LEARN) {
+                const int before = tk.inserts;
+                tk.insert(it_d, F, lane);
+                if (tk.inserts != before) last_insert_hop = tr_expanded;
+            }
         }
         wave_lds_sync();
-        return F;
+    };
+
+    // The next pop, known before the pushes: the smallest key of the
+    // candidates after them is the smaller of (the set's minimum before them,
+    // the best fresh key) -- a push only evicts a maximum, so an old minimum
+    // stays unless every entry equals it, and the best fresh item is accepted
+    // unless every entry already equals it.  The pop then only looks up the
+    // first index of that key.  The adjacency row of the node that holds it
+    // is loaded before the pushes, so its latency hides behind them; with
+    // ties the pop can land on another node, which then loads its own row.
+    int nx_key = 0;
+    int pf_node = -1, nb_pref = -1;
+    auto predict = [&](bool old_ok, int old_min, int old_id, int F) __attribute__((always_inline)) {
+        bool take_new = false;
+        int new_min = 0, new_id = -1;
+        if (F > 0) {
+            const bool in = lane < F;
+            uint32_t u = (uint32_t)it_k;
+            if constexpr (SK) u ^= 0x80000000u;
+            new_min = (int)wave_min_u(in ? u : 0xffffffffu);
+            if constexpr (SK) new_min = (int)((uint32_t)new_min ^ 0x80000000u);
+            new_id = __builtin_amdgcn_readlane(it_id, ffs64(__ballot(in && it_k == new_min)));
+            take_new = !old_ok || new_min < old_min;
+        }
+        if (!take_new && !old_ok) return;  // the candidates end empty
+        nx_key = take_new ? new_min : old_min;
+        pf_node = take_new ? new_id : old_id;
+        nb_pref = lane < P.block ? P.adj[(size_t)pf_node * P.block + lane] : -1;
     };
 
     // Seeds: kernalGraph.h:215-231 -- ids (size_t)p * n / n_seeds, or the
@@ -662,22 +773,26 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
                 sid = (int)(((uint64_t)lane * (uint64_t)P.n) / (uint64_t)P.n_entry);
             }
         }
-        const int F = visit_row(sid, sv, P.n_entry);
+        const int F = gather_row(sid, sv, P.n_entry);
         tr_seed += F;
         tr_visited += F;
+        predict(false, 0, -1, F);
+        push_row(F);
     }
 
     // Main loop: kernalGraph.h:233-289.
     for (;;) {
         if (cand.sz == 0) break;
-        const float c_min = cand.vmin(lane);
+        const int c_min = nx_key;
         const int p = cand.first_eq(c_min, lane);
         const int cn = cand.ids[p];
+        int nb = nb_pref;
+        if (cn != pf_node) nb = lane < P.block ? P.adj[(size_t)cn * P.block + lane] : -1;
         cand.remove_at(p, lane);  // removing the minimum leaves the maximum unless the set empties
         if (cand.sz == 0) cand.mx_ok = false;
         tr_expanded++;
         if (res.sz == res.cap) {
-            const float worst = res.cmax(lane);
+            const int worst = res.cmax(lane);
             if (c_min > worst) {
                 tr_early = 1;
                 break;
@@ -696,13 +811,13 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
             if ((tracing || stopping) && tk.dk < INFINITY && tr_expanded + P.L.interval == next_eval &&
                 tr_expanded >= P.L.min_hops) {
                 const float sc = P.L.dist_scale;
-                const float worst = res.sz == res.cap ? res.cmax(lane) : tk.dk;
+                const int dkk = dist_key<SK>(tk.dk);
                 int closer = 0;  // frontier entries still closer than the k-th best
 #pragma unroll
-                for (int s = 0; s < S; s++) closer += __popcll(__ballot(cand.d[s] < tk.dk));
+                for (int s = 0; s < S; s++) closer += __popcll(__ballot(cand.d[s] != KEY_EMPTY && cand.d[s] < dkk));
                 float f[LM_F];
                 f[0] = (float)tr_expanded * (1.0f / 64.0f);
-                f[1] = (c_min - tk.dk) / sc;
+                f[1] = (key_dist<SK>(c_min) - tk.dk) / sc;
                 f[2] = (tk.dk - tk.d1) / sc;
                 f[3] = (float)(tr_expanded - last_insert_hop) * (1.0f / 16.0f);
                 f[4] = (float)closer / (float)k;
@@ -713,7 +828,6 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
                 for (int i = 0; i < LM_F; i++) f[i] = fminf(fmaxf(f[i], -8.f), 8.f);
                 dk_prev = tk.dk;
                 tk.inserts = 0;
-                (void)worst;
                 if (tracing) {
                     if (n_evals < P.L.max_evals && lane <= LM_F) {
                         float v = tk.dk;
@@ -734,8 +848,18 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
                 }
             }
         }
-        const int nb = lane < P.block ? P.adj[(size_t)cn * P.block + lane] : -1;
-        tr_visited += visit_row(nb, nb >= 0, P.block);
+        // the candidates' minimum without cn, for the next prediction (while
+        // the row is in flight)
+        const bool old_ok = cand.sz > 0;
+        int old_min = 0, old_id = -1;
+        if (old_ok) {
+            old_min = cand.vmin(lane);
+            old_id = cand.ids[cand.first_eq(old_min, lane)];
+        }
+        const int F = gather_row(nb, nb >= 0, P.block);
+        tr_visited += F;
+        predict(old_ok, old_min, old_id, F);
+        push_row(F);
     }
     const int tr_peakr = res.sz;
 
@@ -750,14 +874,11 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
     float* dout = P.dists_out ? P.dists_out + (size_t)qid * P.k_stride : nullptr;
     bool fast = cnt <= 64;
     if (fast) {
-        typename WSet<S>::fvec t = res.d;
-        float od = INFINITY;
+        typename WSet<S, SK>::kvec t = res.d;
+        int od = 0;
         int oi = -1;
         for (int i = 0; i < cnt; i++) {
-            float m = INFINITY;
-#pragma unroll
-            for (int s = 0; s < S; s++) m = fminf(m, t[s]);
-            m = wave_min(m);
+            const int m = WSet<S, SK>::kmin(t);
             int c = 0, p = -1;
 #pragma unroll
             for (int s = 0; s < S; s++) {
@@ -777,29 +898,28 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
             const int slot = p >> 6;
             const bool mine = lane == (p & 63);
 #pragma unroll
-            for (int s = 0; s < S; s++) t[s] = (mine && slot == s) ? __builtin_nanf("") : t[s];
+            for (int s = 0; s < S; s++) t[s] = (mine && slot == s) ? KEY_EMPTY : t[s];
         }
         if (fast && lane < cnt) {
             out[lane] = oi;
-            if (dout) dout[lane] = od;
+            if (dout) dout[lane] = key_dist<SK>(od);
         }
     }
     if (!fast) {
         while (res.sz > k) {
-            const float M = res.vmax(lane);
+            const int M = res.vmax(lane);
             res.remove_at(res.first_eq(M, lane), lane);
         }
         res.mx_ok = false;
         for (int i = 0; i < cnt; i++) {
-            const float m = res.vmin(lane);
+            const int m = res.vmin(lane);
             const int p = res.first_eq(m, lane);
-            float dv;
-            int iv;
-            res.get(p, dv, iv);
+            int kv, iv;
+            res.get(p, kv, iv);
             res.remove_at(p, lane);
             if (lane == 0) {
                 out[i] = iv;
-                if (dout) dout[i] = dv;
+                if (dout) dout[i] = key_dist<SK>(kv);
             }
         }
     }
@@ -837,9 +957,15 @@ constexpr int STOP_LDS_U64 = (LM_STOP_FLOATS + 1) / 2;
 template <int S>
 __host__ __device__ constexpr size_t wave_lds_u64(int bloom_words) { return (size_t)bloom_words + 64 + 64 * S; }
 
+// Register budget: with two 16-byte chunks per lane the plain search fits in
+// 80 VGPRs, which lets 6 waves share a SIMD (4 at the default allocation).
+// Larger rows and the learned stop keep the compiler's own choice.
+__host__ __device__ constexpr int waves_per_simd(int qch, int learn) { return qch <= 2 && !learn ? 6 : 1; }
+
 // One wave per query (the reference's one block per query, kernalGraph.h:354).
 template <int METRIC, int S, int G, int QCH, int LEARN>
-__global__ void __launch_bounds__(512) search_kernel(SearchArgs P) {
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(waves_per_simd(QCH, LEARN))))
+search_kernel(SearchArgs P) {
     extern __shared__ unsigned long long smem[];
     unsigned long long* waves_lds = smem;
     const float* stop_lds = nullptr;
@@ -897,7 +1023,8 @@ __device__ __forceinline__ int64_t resident_claim(const ResidentQueue& Q) {
 // resident queue the waves serve the request ring instead (resident_claim),
 // publishing each request's results before its done flag.
 template <int METRIC, int S, int G, int QCH, int LEARN>
-__global__ void __launch_bounds__(512) persistent_kernel(SearchArgs P) {
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(waves_per_simd(QCH, LEARN))))
+persistent_kernel(SearchArgs P) {
     extern __shared__ unsigned long long smem[];
     unsigned long long* waves_lds = smem;
     const float* stop_lds = nullptr;
@@ -979,7 +1106,16 @@ static hipError_t launch_g(const SearchArgs& a, int G, int qch, const LaunchShap
     if constexpr (METRIC == METRIC_HAMMING) {
         return launch_one<METRIC, S, 2, 1>(a, wpb, st);
     } else {
-        if (qch == 4) {
+        if (qch == 2) {
+            switch (G) {
+            case 2: return launch_one<METRIC, S, 2, 2>(a, wpb, st);
+            case 4: return launch_one<METRIC, S, 4, 2>(a, wpb, st);
+            case 8: return launch_one<METRIC, S, 8, 2>(a, wpb, st);
+            case 16: return launch_one<METRIC, S, 16, 2>(a, wpb, st);
+            case 32: return launch_one<METRIC, S, 32, 2>(a, wpb, st);
+            case 64: return launch_one<METRIC, S, 64, 2>(a, wpb, st);
+            }
+        } else if (qch == 4) {
             switch (G) {
             case 2: return launch_one<METRIC, S, 2, 4>(a, wpb, st);
             case 4: return launch_one<METRIC, S, 4, 4>(a, wpb, st);

@@ -13,10 +13,11 @@ pytestmark = pytest.mark.gpu
 
 
 def lanes_for(dim):
-    """The lane-group size the library picks for a dim (row chunks of 4 floats)."""
+    """The lane-group size the library picks for a dim: the smallest power of
+    two (2..64) whose lanes hold the row's 4-float chunks two per lane."""
     d4 = (dim + 3) // 4
     g = 2
-    while g < 64 and g * 4 < d4:
+    while g < 64 and g * 2 < d4:
         g *= 2
     return g
 
