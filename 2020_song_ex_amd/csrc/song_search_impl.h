@@ -554,8 +554,11 @@ __device__ __forceinline__ void hamming_distances(const ulonglong2* __restrict__
 // The stop network's weights in the kernel's layout (the host converts them
 // once, stop_weights_to_kernel_layout): w1 [64][8], b1 [64], w2 interleaved so
 // lane j reads 16 conflict-free, coalesced float4 (element (j, i) at
-// (i/4)*256 + j*4 + i%4), b2 [64], w3 [64].  Each workgroup stages them in
-// LDS once.
+// (i/4)*256 + j*4 + i%4), b2 [64], w3 [64].  The search reads them from
+// global memory: 19.2 KB, shared by every wave of a compute unit through its
+// L1, so LDS holds only the waves' own state (staged per workgroup, the
+// weights would leave LDS room for two 8-wave workgroups per CU, 16 waves).
+// stage_stop_weights serves the stand-alone inference kernel.
 constexpr int SW_B1 = LM_H * LM_F;         // b1 [64]
 constexpr int SW_W2 = SW_B1 + LM_H;        // w2 [64][64], interleaved
 constexpr int SW_B2 = SW_W2 + LM_H * LM_H; // b2 [64]
@@ -565,14 +568,22 @@ __device__ __forceinline__ void stage_stop_weights(const float* __restrict__ src
     for (int i = threadIdx.x; i < LM_STOP_FLOATS; i += blockDim.x) lds[i] = src[i];
 }
 
-// logit of the stop network for wave-uniform features f[0..7].  hbuf: 64
-// floats of this wave's LDS; the hidden layer goes through it so that every
-// lane reads it back as broadcast float4s (no per-unit lane reads).
-__device__ __forceinline__ float stop_logit(const float* lds, const float (&f)[LM_F], float b3, int lane,
-                                            float* hbuf) {
-    const float4 wa = *(const float4*)(lds + lane * LM_F);
-    const float4 wb = *(const float4*)(lds + lane * LM_F + 4);
-    float h = lds[SW_B1 + lane];
+// logit of the stop network for wave-uniform features f[0..7].  W: the
+// weights in the kernel layout (global memory or LDS).  hbuf: 64 floats of
+// this wave's LDS; the hidden layer goes through it so that every lane reads
+// it back as broadcast float4s (no per-unit lane reads).  The weights' base
+// address passes through an empty asm so that the compiler forms each load's
+// address here, from the wave-uniform base plus the lane's offset, instead of
+// hoisting 16 per-lane 64-bit addresses out of the search loop; the second
+// layer's loop stays rolled (one row block in flight), which keeps the
+// search within its register budget.
+__device__ __forceinline__ float stop_logit(const float* __restrict__ W, const float (&f)[LM_F], float b3,
+                                            int lane, float* hbuf) {
+    asm volatile("" : "+s"(W));
+    const float4* w1 = (const float4*)W;
+    const float4 wa = w1[2 * lane];
+    const float4 wb = w1[2 * lane + 1];
+    float h = W[SW_B1 + lane];
     h = __builtin_fmaf(wa.x, f[0], h);
     h = __builtin_fmaf(wa.y, f[1], h);
     h = __builtin_fmaf(wa.z, f[2], h);
@@ -583,10 +594,11 @@ __device__ __forceinline__ float stop_logit(const float* lds, const float (&f)[L
     h = __builtin_fmaf(wb.w, f[7], h);
     hbuf[lane] = fmaxf(h, 0.f);
     wave_lds_sync();
-    float acc = lds[SW_B2 + lane];
-#pragma unroll
+    float acc = W[SW_B2 + lane];
+    const float4* w2 = (const float4*)(W + SW_W2);
+#pragma unroll 1
     for (int i4 = 0; i4 < 16; i4++) {
-        const float4 w = *(const float4*)(lds + SW_W2 + i4 * 256 + lane * 4);
+        const float4 w = w2[i4 * 64 + lane];
         const float4 x = *(const float4*)(hbuf + 4 * i4);
         acc = __builtin_fmaf(w.x, x.x, acc);
         acc = __builtin_fmaf(w.y, x.y, acc);
@@ -595,7 +607,7 @@ __device__ __forceinline__ float stop_logit(const float* lds, const float (&f)[L
     }
     wave_lds_sync();
     acc = fmaxf(acc, 0.f);
-    return wave_sum(acc * lds[SW_W3 + lane]) + b3;
+    return wave_sum(acc * W[SW_W3 + lane]) + b3;
 }
 
 // The k smallest distances seen so far, ascending in lanes 0..k-1 (k <= 64):
@@ -681,8 +693,9 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
     int tr_seed = 0, tr_visited = 0, tr_expanded = 0;
     int tr_fpush = 0, tr_rupd = 0, tr_peakf = 0, tr_early = 0;
     // learned termination state (k <= 64 on this path)
-    const bool tracing = LEARN && P.L.trace != nullptr;
-    const bool stopping = LEARN && P.L.use_stop && !tracing;
+    // LEARN 1: the learned stop; LEARN 2: record its features instead
+    const bool tracing = LEARN == 2 && P.L.trace != nullptr;
+    const bool stopping = LEARN == 1 && P.L.use_stop;
     TopK tk;
     tk.init(k);
     float dk_prev = INFINITY;
@@ -828,7 +841,7 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
                 for (int i = 0; i < LM_F; i++) f[i] = fminf(fmaxf(f[i], -8.f), 8.f);
                 dk_prev = tk.dk;
                 tk.inserts = 0;
-                if (tracing) {
+                if constexpr (LEARN == 2) {
                     if (n_evals < P.L.max_evals && lane <= LM_F) {
                         float v = tk.dk;
 #pragma unroll
@@ -953,14 +966,16 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
 
 // LDS of a workgroup: [stop weights when learned] then per wave
 // bloom[W] | stage_id[64] | stage_d[64] | set ids [2 x 64 * S] (8-byte words).
-constexpr int STOP_LDS_U64 = (LM_STOP_FLOATS + 1) / 2;
 template <int S>
 __host__ __device__ constexpr size_t wave_lds_u64(int bloom_words) { return (size_t)bloom_words + 64 + 64 * S; }
 
 // Register budget: with two 16-byte chunks per lane the plain search fits in
-// 80 VGPRs, which lets 6 waves share a SIMD (4 at the default allocation).
-// Larger rows and the learned stop keep the compiler's own choice.
-__host__ __device__ constexpr int waves_per_simd(int qch, int learn) { return qch <= 2 && !learn ? 6 : 1; }
+// 80 VGPRs, so 6 waves share a SIMD (4 at the compiler's own choice), and the
+// search with the learned stop fits in 96 (5 waves).  Larger rows and the
+// feature-recording search keep the compiler's own choice.
+__host__ __device__ constexpr int waves_per_simd(int qch, int learn) {
+    return qch > 2 ? 1 : learn == 0 ? 6 : learn == 1 ? 5 : 1;
+}
 
 // One wave per query (the reference's one block per query, kernalGraph.h:354).
 template <int METRIC, int S, int G, int QCH, int LEARN>
@@ -968,15 +983,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(waves_
 search_kernel(SearchArgs P) {
     extern __shared__ unsigned long long smem[];
     unsigned long long* waves_lds = smem;
-    const float* stop_lds = nullptr;
-    if constexpr (LEARN) {
-        if (P.L.use_stop && !P.L.trace) {
-            stage_stop_weights(P.L.s_weights, (float*)smem);
-            __syncthreads();
-            stop_lds = (const float*)smem;
-            waves_lds = smem + STOP_LDS_U64;
-        }
-    }
+    const float* stop_lds = LEARN == 1 ? P.L.s_weights : nullptr;
     const int wave = threadIdx.x >> 6;
     const int pos = blockIdx.x * (blockDim.x >> 6) + wave;
     if (pos >= P.nq) return;
@@ -1027,15 +1034,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(waves_
 persistent_kernel(SearchArgs P) {
     extern __shared__ unsigned long long smem[];
     unsigned long long* waves_lds = smem;
-    const float* stop_lds = nullptr;
-    if constexpr (LEARN) {
-        if (P.L.use_stop && !P.L.trace) {
-            stage_stop_weights(P.L.s_weights, (float*)smem);
-            __syncthreads();
-            stop_lds = (const float*)smem;
-            waves_lds = smem + STOP_LDS_U64;
-        }
-    }
+    const float* stop_lds = LEARN == 1 ? P.L.s_weights : nullptr;
     const int wave = threadIdx.x >> 6;
     const int lane = lane_id();
     unsigned long long* lds = waves_lds + (size_t)wave * wave_lds_u64<S>(P.bloom_words);
@@ -1077,8 +1076,10 @@ template <int METRIC, int S, int G, int QCH, int LEARN>
 static hipError_t launch_impl(const SearchArgs& a, const LaunchShape& ls, hipStream_t st) {
     const int threads = 64 * ls.waves_per_block;
     size_t lds = (size_t)ls.waves_per_block * wave_lds_u64<S>(a.bloom_words) * 8;
-    if (LEARN && a.L.use_stop && !a.L.trace) lds += (size_t)STOP_LDS_U64 * 8;
-    if (kResidentServing && a.resident) {  // the ring's head persists across launches
+    if constexpr (LEARN == 2) {  // recording features: one launch shape (same results in every mode)
+        const int blocks = (a.nq + ls.waves_per_block - 1) / ls.waves_per_block;
+        hipLaunchKernelGGL((search_kernel<METRIC, S, G, QCH, 2>), dim3(blocks), dim3(threads), lds, st, a);
+    } else if (kResidentServing && a.resident) {  // the ring's head persists across launches
         hipLaunchKernelGGL((persistent_kernel<METRIC, S, G, QCH, LEARN>), dim3(ls.persistent_blocks), dim3(threads),
                            lds, st, a);
     } else if (a.queue_head) {
@@ -1097,7 +1098,10 @@ static hipError_t launch_impl(const SearchArgs& a, const LaunchShape& ls, hipStr
 template <int METRIC, int S, int G, int QCH>
 static hipError_t launch_one(const SearchArgs& a, const LaunchShape& ls, hipStream_t st) {
     if (!a.learned) return launch_impl<METRIC, S, G, QCH, 0>(a, ls, st);
-    if constexpr (QCH <= 4) return launch_impl<METRIC, S, G, QCH, 1>(a, ls, st);
+    if constexpr (QCH <= 4) {
+        if (a.L.trace) return launch_impl<METRIC, S, G, QCH, 2>(a, ls, st);
+        return launch_impl<METRIC, S, G, QCH, 1>(a, ls, st);
+    }
     return hipErrorInvalidValue;
 }
 
