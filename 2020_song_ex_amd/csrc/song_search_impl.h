@@ -388,25 +388,38 @@ struct WSet {
         return 0;
     }
     __device__ __forceinline__ int first_eq(int v, int lane) const { return first_eq_of(d, v); }
+    // Slot access at a wave-uniform runtime index: a compare-and-branch per
+    // slot up to the one addressed, which then costs one VALU operation --
+    // a select chain would cost an operation per slot plus scalar mask work.
+    // (The empty asm keeps the compiler from folding the branches back into
+    // such a chain.)
     __device__ __forceinline__ int get_d(int idx) const {
         const int slot = idx >> 6, ln = idx & 63;
-        int dd = d[0];
+        int r = KEY_EMPTY;
 #pragma unroll
-        for (int s = 1; s < S; s++)
-            if (slot == s) dd = d[s];
-        return __builtin_amdgcn_readlane(dd, ln);
+        for (int s = 0; s < S; s++) {
+            if (slot == s) {
+                asm volatile("");
+                r = __builtin_amdgcn_readlane(d[s], ln);
+            }
+        }
+        return r;
     }
     __device__ __forceinline__ void get(int idx, int& kv, int& iv) const {
         kv = get_d(idx);
         iv = ids[idx];
     }
-    // Branch-free on the keys: every lane selects, only lane idx%64 in slot
-    // idx/64 changes; one lane writes the id.
+    // Only lane idx%64 of slot idx/64 changes; one lane writes the id.
     __device__ __forceinline__ void put_d(int idx, int kv, int lane) {
         const int slot = idx >> 6;
         const bool mine = lane == (idx & 63);
 #pragma unroll
-        for (int s = 0; s < S; s++) d[s] = (mine && slot == s) ? kv : d[s];
+        for (int s = 0; s < S; s++) {
+            if (slot == s) {
+                asm volatile("");
+                d[s] = mine ? kv : d[s];
+            }
+        }
     }
     __device__ __forceinline__ void put(int idx, int kv, int iv, int lane) {
         put_d(idx, kv, lane);
@@ -421,6 +434,24 @@ struct WSet {
         put(p, lk, li, lane);
         put_d(last, KEY_EMPTY, lane);
         sz--;
+    }
+    // Replace the first entry holding key M (present in the set) by (kv, iv):
+    // the first-index search and the write in one pass, so the write happens
+    // in the branch of the slot found, at a slot fixed at compile time.
+    __device__ __forceinline__ void replace_first(int M, int kv, int iv, int lane) {
+        bool done = false;
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+            if (!done) {
+                const uint64_t b = __ballot(d[s] == M);
+                if (b) {
+                    const int l = ffs64(b);
+                    d[s] = lane == l ? kv : d[s];
+                    if (lane == 0) ids[s * 64 + l] = iv;
+                    done = true;
+                }
+            }
+        }
     }
     // Push items it_k/it_id held by lanes 0..F-1, in lane order, with the
     // Candidates::push / Results::try_push rule (gpuHeap.h:23-36, 78-87).
@@ -448,8 +479,7 @@ struct WSet {
                 const int r = ffs64(m);
                 const int kv = __builtin_amdgcn_readlane(it_k, r);
                 const int iv = __builtin_amdgcn_readlane(it_id, r);
-                const int p = first_eq(M, lane);
-                put(p, kv, iv, lane);
+                replace_first(M, kv, iv, lane);
                 accepted++;
                 mx_ok = false;
                 start = r + 1;
