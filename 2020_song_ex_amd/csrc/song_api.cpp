@@ -104,7 +104,7 @@ struct Plan {
     int learned;  // bit 0 entry, bit 1 stop, bit 2 cost-ordered queue, bit 3 keep the reference seeds
     int n_entry;  // seed-row length: n_seeds, or 2 * n_seeds with bit 3
     float stop_threshold;
-    int stop_min_hops, stop_interval, stop_max_hops, stop_confirm;
+    int stop_min_hops, stop_interval, stop_max_hops, stop_confirm, result_cap;
     song::LaunchShape shape;
 };
 
@@ -164,6 +164,13 @@ int make_plan(const song_index* ix, int32_t nq, int32_t k, int32_t ef, const son
     if (o->stop_max_hops < 0) return fail(SONG_ERR_ARG, "stop_max_hops must be >= 0");
     p->stop_confirm = o->stop_confirm ? o->stop_confirm : 1;
     if (o->stop_confirm < 0 || o->stop_confirm > 64) return fail(SONG_ERR_ARG, "stop_confirm must be in [0, 64]");
+    p->result_cap = o->result_cap;
+    if (o->result_cap < 0) return fail(SONG_ERR_ARG, "result_cap must be >= 0");
+    if (o->result_cap > 0 && o->result_cap < ef) {
+        if (!(o->learned & 2)) return fail(SONG_ERR_ARG, "result_cap below the queue size needs the learned stop (bit 1)");
+        if (o->stop_max_hops <= 0) return fail(SONG_ERR_ARG, "result_cap below the queue size needs stop_max_hops > 0");
+        if (o->result_cap < k) return fail(SONG_ERR_ARG, "result_cap must be >= k");
+    }
     if (o->learned < 0 || o->learned > 15) return fail(SONG_ERR_ARG, "learned must be 0..15");
     if ((o->learned & 8) && !(o->learned & 1)) return fail(SONG_ERR_ARG, "learned bit 3 needs the entry network (bit 0)");
     p->n_entry = (o->learned & 8) ? 2 * p->n_seeds : p->n_seeds;
@@ -212,6 +219,7 @@ song::SearchArgs make_args(const song_index* ix, const Plan& p, const void* dq, 
         a.L.interval = p.stop_interval;
         a.L.max_hops = p.stop_max_hops;
         a.L.confirm = p.stop_confirm;
+        a.L.result_cap = p.result_cap;
         a.L.trace = nullptr;
         a.L.trace_count = nullptr;
         a.L.final_dk = nullptr;

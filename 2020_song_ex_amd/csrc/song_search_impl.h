@@ -713,8 +713,14 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
 
     constexpr bool SK = METRIC == METRIC_IP;
     WSet<S, SK> cand, res;
+    // Learned stop with a result cap below ef: the result set keeps only
+    // result_cap (>= k) entries, and the reference's early stop -- which
+    // compares with the worst of ef results -- is left to the stop network
+    // and the hop cap (the host requires one).  The traversal (candidates,
+    // Bloom) does not depend on the result set, and its k best are the same.
+    const bool small_res = LEARN == 1 && P.L.result_cap > 0 && P.L.result_cap < ef;
     cand.init(ef, set_ids);
-    res.init(ef, set_ids + 64 * S);
+    res.init(small_res ? max(P.L.result_cap, k) : ef, set_ids + 64 * S);
     // QueryTrace, telemetry.h:5-18 (wave-uniform counters)
     // (distance_evals = visited, candidate_nodes = visited - seeds,
     // neighbor_slots = block per expansion that read its row, and the result
@@ -834,7 +840,7 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
         cand.remove_at(p, lane);  // removing the minimum leaves the maximum unless the set empties
         if (cand.sz == 0) cand.mx_ok = false;
         tr_expanded++;
-        if (res.sz == res.cap) {
+        if (!small_res && res.sz == res.cap) {
             const int worst = res.cmax(lane);
             if (c_min > worst) {
                 tr_early = 1;

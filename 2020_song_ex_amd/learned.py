@@ -227,10 +227,12 @@ def kmeans_pivots(index: Index, base, n_pivots=N_PIVOTS, sample=50_000, iters=12
 
 def fit(index: Index, base, train_queries, *, k=10, ef=256, n_seeds=4, bloom_bits=0, stop_min_hops=8,
         stop_interval=2, max_evals=128, entry_epochs=20, stop_epochs=4, cost_epochs=20, cost_ef=None,
-        cost_threshold=2.5, pivot_pool=1.0, seed=0, log=None):
+        cost_threshold=2.5, pivot_pool=1.0, seed=0, log=None, trace_entry=True, trace_ef=None):
     """Train the three networks for this index; returns the LearnedModel (already set on the index).
     The cost network learns the hop counts of the learned search (entry + stop at cost_threshold) at
-    queue size cost_ef (default ef)."""
+    queue size cost_ef (default ef).  The stop network learns from traces of the search at queue
+    size trace_ef (default ef), from the learned entry points (trace_entry) or the reference seeds:
+    the features depend on both, so they are best taken as the stop will be used."""
     say = log or (lambda *_: None)
     metric = index.metric
     proj_dim = 256 if metric == N.METRIC_HAMMING else index.row_stride
@@ -279,17 +281,17 @@ def fit(index: Index, base, train_queries, *, k=10, ef=256, n_seeds=4, bloom_bit
     model.dist_scale = scale
     model.set_on(index)
 
-    # stop traces with the learned entry
+    # stop traces (from the learned entry points or the reference seeds)
     dq = index.queries_to_device(train_queries)
     d_tr = N.DeviceBuffer(nq * max_evals * (N_FEATURES + 1) * 4)
     d_cnt = N.DeviceBuffer(nq * 4)
     d_fin = N.DeviceBuffer(nq * 4)
     o = N.make_opts(bloom_bits, 0, n_seeds, 0, 0, 0, 0)
-    o.learned = 1
+    o.learned = 1 if trace_entry else 0
     o.stop_min_hops = stop_min_hops
     o.stop_interval = stop_interval
     L = _bind()
-    N.check(L.song_learn_trace(index.handle, dq.ptr, nq, k, ef, ctypes.byref(o), max_evals, d_tr.ptr,
+    N.check(L.song_learn_trace(index.handle, dq.ptr, nq, k, trace_ef or ef, ctypes.byref(o), max_evals, d_tr.ptr,
                                d_cnt.ptr, d_fin.ptr), "song_learn_trace")
     tr = d_tr.download(np.empty((nq, max_evals, N_FEATURES + 1), np.float32))
     cnt = d_cnt.download(np.empty(nq, np.int32))

@@ -289,9 +289,12 @@ def main():
             log(f"learned model trained in {time.time() - t0:.1f}s")
             if args.model:
                 model.save(args.model)
-        grid = {"ef": [384, 448, 512], "learned": [2], "stop_threshold": [2.0, 2.5, 3.0, 3.5],
-                "stop_confirm": [1, 2, 3], "stop_max_hops": [0, 600], "stop_min_hops": [16], "stop_interval": [4],
-                "mode": [1], "waves_per_block": [4]}
+        # result_cap: the result set holds 16 or 32 (>= k) instead of the queue size, and the search
+        # ends by the stop network or the hop cap, not by the reference's test against the worst
+        # of ef results (see include/song.h) -- which lets a short queue search long enough
+        grid = {"ef": [64, 96, 128, 160, 192, 256], "learned": [2], "stop_threshold": [1.5, 1.75, 2.0, 2.5],
+                "stop_confirm": [1, 2, 3], "stop_max_hops": [600], "result_cap": [16, 32], "stop_min_hops": [16],
+                "stop_interval": [4], "mode": [1], "waves_per_block": [4]}
         given = json.loads(args.learned_config) if args.learned_config else stored.get("learned")
         if given:
             grid = {kk: [v] for kk, v in given.items()}

@@ -133,7 +133,16 @@ typedef struct song_search_opts {
                                 expanded this many nodes (0 = no cap) */
     int32_t stop_confirm;    /* stop after this many consecutive consults above
                                 the threshold (0 = 1) */
-    int32_t reserved[3];
+    int32_t result_cap;      /* with learned bit 1 and a hop cap: result-set
+                                capacity, k <= result_cap < ef (0 = ef).  The
+                                search then ends by the stop network, the hop
+                                cap or an empty frontier, without the
+                                reference's "best candidate worse than the
+                                worst of the ef results" test, which needs an
+                                ef-sized result set (kernalGraph.h:239-241);
+                                the k best are kept the same way.  No
+                                reference counterpart. */
+    int32_t reserved[2];
 } song_search_opts;
 
 enum { SONG_MODE_PLAIN = 0, SONG_MODE_PERSISTENT = 1 };

@@ -76,7 +76,7 @@ def lib():
         L.or_search_capped.argtypes = [_P, _P, _I64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P,
                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, _I64, ctypes.c_int,
                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P,
-                                       ctypes.c_int, ctypes.c_int, _P, _P, _P]
+                                       ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P, _P]
         L.or_search_capped.restype = ctypes.c_int
         L.or_cpu_search.argtypes = [_P, _P, _I64, ctypes.c_int, ctypes.c_int, _P, ctypes.c_int,
                                     ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P]
@@ -121,7 +121,8 @@ def ref():
 # --------------------------------------------------------------------------- oracle calls
 
 def search(data, adj, queries, k, ef, *, block, metric="l2", bloom_bits=1 << 14, n_seeds=4,
-           group=8, ref_order=False, exact_visited=False, unbounded=False, entries=None, max_hops=0):
+           group=8, ref_order=False, exact_visited=False, unbounded=False, entries=None, max_hops=0,
+           result_cap=0):
     """Re-execute the reference GPU search (kernalGraph.h:184-299) on the CPU.
 
     ``data``: float32 [n, dim] (l2/ip) or uint64 [n, 4] (hamming).
@@ -147,7 +148,8 @@ def search(data, adj, queries, k, ef, *, block, metric="l2", bloom_bits=1 << 14,
         ne = entries.shape[1]
     rc = lib().or_search_capped(_ptr(data), _ptr(adj), n, dim, block, m, _ptr(queries), nq, k, ef,
                                 bloom_bits, n_seeds, group, int(ref_order), int(exact_visited),
-                                int(unbounded), _ptr(entries), ne, int(max_hops), _ptr(ids), _ptr(d), _ptr(tr))
+                                int(unbounded), _ptr(entries), ne, int(max_hops), int(result_cap), _ptr(ids),
+                                _ptr(d), _ptr(tr))
     if rc != 0:
         raise ValueError("or_search rejected its arguments")
     return ids, d, tr

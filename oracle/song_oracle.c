@@ -362,6 +362,9 @@ typedef struct {
     int unbounded;     /* knob: frontier never evicts (cap = n + n_seeds)     */
     int max_hops;      /* learned search's hop cap (0 = none; no reference
                           counterpart): stop once expanded > max_hops        */
+    int res_cap;       /* result-set capacity: ef with the reference's early
+                          stop; below ef (learned search's result_cap, no
+                          reference counterpart) without that test           */
 } or_params;
 
 /* Seeds at the entry of the search, kernalGraph.h:215-231: n_seeds evenly
@@ -406,7 +409,7 @@ static void search_one(const or_space *sp, const or_params *p, or_work *w, const
     memset(tr, 0, TR_N * sizeof(int32_t));
     int frontier_cap = p->unbounded ? (int)(p->n + n_entry + 1) : p->ef;
     set_init(&w->cand, w->cd, w->ci, frontier_cap);
-    set_init(&w->res, w->rd, w->ri, p->ef);
+    set_init(&w->res, w->rd, w->ri, p->res_cap);
     if (p->exact_visited) memset(w->vis, 0, (size_t)p->n);
     else memset(w->bloom, 0, (size_t)p->bloom_words * 8);
 
@@ -432,7 +435,7 @@ static void search_one(const or_space *sp, const or_params *p, or_work *w, const
         float c_min;
         int cn = set_pop_min(&w->cand, &c_min);
         tr[TR_EXPANDED]++;
-        if (w->res.sz == w->res.cap && c_min > set_worst(&w->res)) {
+        if (p->res_cap == p->ef && w->res.sz == w->res.cap && c_min > set_worst(&w->res)) {
             tr[TR_EARLY] = 1;
             break;
         }
@@ -480,7 +483,7 @@ static int64_t row_elems(const or_space *sp) { return sp->metric == OR_METRIC_HA
 int or_search_capped(const void *data, const int32_t *adj, int64_t n, int dim, int block, int metric,
                      const void *queries, int nq, int k, int ef, int64_t bloom_bits, int n_seeds, int G,
                      int ref_order, int exact_visited, int unbounded, const int32_t *entries, int n_entry,
-                     int max_hops, int32_t *ids_out, float *dists_out, int32_t *trace_out);
+                     int max_hops, int result_cap, int32_t *ids_out, float *dists_out, int32_t *trace_out);
 
 int or_search(const void *data, const int32_t *adj, int64_t n, int dim, int block, int metric,
               const void *queries, int nq, int k, int ef, int64_t bloom_bits, int n_seeds, int G,
@@ -488,22 +491,30 @@ int or_search(const void *data, const int32_t *adj, int64_t n, int dim, int bloc
               int32_t *ids_out, float *dists_out, int32_t *trace_out)
 {
     return or_search_capped(data, adj, n, dim, block, metric, queries, nq, k, ef, bloom_bits, n_seeds, G,
-                            ref_order, exact_visited, unbounded, entries, n_entry, 0, ids_out, dists_out,
+                            ref_order, exact_visited, unbounded, entries, n_entry, 0, 0, ids_out, dists_out,
                             trace_out);
 }
 
-/* or_search with the learned search's hop cap (max_hops > 0). */
+/* or_search with the learned search's hop cap (max_hops > 0) and result-set
+ * capacity (result_cap: 0 = ef; k <= result_cap < ef replaces the reference's
+ * "best candidate worse than the worst result" test, which needs an ef-sized
+ * result set, by the hop cap, which must then be set). */
 int or_search_capped(const void *data, const int32_t *adj, int64_t n, int dim, int block, int metric,
                      const void *queries, int nq, int k, int ef, int64_t bloom_bits, int n_seeds, int G,
                      int ref_order, int exact_visited, int unbounded, const int32_t *entries, int n_entry,
-                     int max_hops, int32_t *ids_out, float *dists_out, int32_t *trace_out)
+                     int max_hops, int result_cap, int32_t *ids_out, float *dists_out, int32_t *trace_out)
 {
-    if (max_hops < 0) return -1;
+    if (max_hops < 0 || result_cap < 0) return -1;
     if (n < 1 || k < 1 || ef < 1 || block < 1 || nq < 0 || G < 1 || G > 64) return -1;
     if (!exact_visited && (bloom_bits < 64 || (bloom_bits % 64) != 0)) return -1;
     if (ef < k) ef = k; /* kernalGraph.h:315 (the reference also caps ef at 50; lifted here) */
+    int res_cap = ef;
+    if (result_cap > 0 && result_cap < ef) {
+        if (result_cap < k || max_hops == 0) return -1;
+        res_cap = result_cap;
+    }
     or_space sp = {metric, dim, G, ref_order, (const float *)data, (const uint64_t *)data};
-    or_params p = {adj, n, block, k, ef, bloom_bits / 64, n_seeds, exact_visited, unbounded, max_hops};
+    or_params p = {adj, n, block, k, ef, bloom_bits / 64, n_seeds, exact_visited, unbounded, max_hops, res_cap};
     int ne = entries ? n_entry : n_seeds;
     int64_t fcap = unbounded ? n + ne + 1 : ef;
     or_work w;

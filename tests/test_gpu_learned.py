@@ -187,6 +187,39 @@ def test_hop_cap_matches_oracle(song, oracle, setup, cap):
     assert (r.trace[:, 11] == 3).any()
 
 
+@pytest.mark.parametrize("rcap", [10, 32])
+def test_result_cap_matches_oracle(song, oracle, setup, rcap):
+    """A result cap below the queue size, with the stop network never firing:
+    the search runs to the hop cap (or an empty frontier) without the reference's
+    early stop and keeps a result set of rcap entries -- the oracle's search with
+    the same result capacity, bit for bit."""
+    base, q, adj, idx, model = setup
+    r = idx.search(q, 10, 128, learned=2, stop_threshold=1e30, stop_min_hops=1, stop_max_hops=40, result_cap=rcap)
+    oi, od, ot = oracle.search(base, adj, q, 10, 128, block=32, bloom_bits=1 << 14, group=4, max_hops=40,
+                               result_cap=rcap)
+    np.testing.assert_array_equal(r.ids, oi)
+    np.testing.assert_array_equal(r.dists.view(np.uint32), od.view(np.uint32))
+    np.testing.assert_array_equal(r.trace, ot)
+    assert (r.trace[:, 9] <= rcap).all()
+
+
+def test_result_cap_options(song, setup):
+    """result_cap = queue size is the plain rule; below it the learned stop and a hop
+    cap are required, and it must hold k."""
+    base, q, adj, idx, model = setup
+    kw = dict(learned=2, stop_threshold=2.0, stop_min_hops=8, stop_interval=2, stop_max_hops=200)
+    a = idx.search(q, 10, 128, **kw)
+    b = idx.search(q, 10, 128, result_cap=128, **kw)
+    np.testing.assert_array_equal(a.ids, b.ids)
+    np.testing.assert_array_equal(a.trace, b.trace)
+    c = idx.search(q, 10, 128, result_cap=64, **kw)
+    assert c.trace[:, 7].sum() < a.trace[:, 7].sum()  # fewer result-set updates
+    for bad in (dict(result_cap=64), dict(result_cap=64, learned=2, stop_threshold=2.0),
+                dict(result_cap=5, **kw), dict(result_cap=-1, **kw)):
+        with pytest.raises(song.SongError, match="result_cap"):
+            idx.search(q[:4], 10, 128, **bad)
+
+
 def test_confirmed_stop_runs_longer(song, setup):
     """Needing two consecutive stop votes never stops earlier than one vote,
     and one confirmation is the default."""

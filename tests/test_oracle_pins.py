@@ -296,3 +296,29 @@ def test_bruteforce_matches_numpy(oracle):
         order = np.lexsort((np.arange(800), dist))[:7]
         np.testing.assert_array_equal(ids[i], order)
         np.testing.assert_array_equal(d[i], dist[order])
+
+
+@pytest.mark.parametrize("rcap", [10, 24])
+def test_result_cap(oracle, song, datasets, rcap):
+    """A result set smaller than the queue (the learned search's result_cap)
+    leaves the traversal alone: with a hop cap that every query reaches, the
+    same nodes are expanded and visited and the same k best come out; only the
+    result set's own counters (updates, peak size) change, and it takes no more
+    updates than the ef-sized set.  Below k, without the hop cap, or below zero
+    it is refused."""
+    cfg = datasets.config1()
+    adj = song.nsw_build(cfg["base"], block=16, exact_order=False)
+    kw = dict(block=16, bloom_bits=1 << 15, group=8, max_hops=12)
+    fi, fd, ft = oracle.search(cfg["base"], adj, cfg["queries"], 10, 64, **kw)
+    assert (ft[:, 11] == 3).all()  # every query reaches the hop cap
+    ids, d, tr = oracle.search(cfg["base"], adj, cfg["queries"], 10, 64, result_cap=rcap, **kw)
+    np.testing.assert_array_equal(d, fd)
+    ties = (np.diff(fd, axis=1) == 0).any(1) | (fd[:, -1:] == 0).any(1)
+    np.testing.assert_array_equal(ids[~ties], fi[~ties])
+    same = [c for c in range(12) if c not in (7, 9)]
+    np.testing.assert_array_equal(tr[:, same], ft[:, same])
+    assert (tr[:, 9] == rcap).all() and (tr[:, 7] <= ft[:, 7]).all()
+    for bad in (dict(result_cap=9, max_hops=12), dict(result_cap=24), dict(result_cap=-1, max_hops=12)):
+        with pytest.raises(ValueError):
+            oracle.search(cfg["base"], adj, cfg["queries"][:2], 10, 64, block=16, bloom_bits=1 << 15, group=8,
+                          **bad)
