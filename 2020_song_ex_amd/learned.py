@@ -227,12 +227,15 @@ def kmeans_pivots(index: Index, base, n_pivots=N_PIVOTS, sample=50_000, iters=12
 
 def fit(index: Index, base, train_queries, *, k=10, ef=256, n_seeds=4, bloom_bits=0, stop_min_hops=8,
         stop_interval=2, max_evals=128, entry_epochs=20, stop_epochs=4, cost_epochs=20, cost_ef=None,
-        cost_threshold=2.5, pivot_pool=1.0, seed=0, log=None, trace_entry=True, trace_ef=None):
+        cost_threshold=2.5, pivot_pool=1.0, seed=0, log=None, trace_entry=True, trace_ef=None,
+        cost_search=None):
     """Train the three networks for this index; returns the LearnedModel (already set on the index).
     The cost network learns the hop counts of the learned search (entry + stop at cost_threshold) at
     queue size cost_ef (default ef).  The stop network learns from traces of the search at queue
     size trace_ef (default ef), from the learned entry points (trace_entry) or the reference seeds:
-    the features depend on both, so they are best taken as the stop will be used."""
+    the features depend on both, so they are best taken as the stop will be used.  cost_search:
+    the search options (Index.search keywords) of the run whose hop counts the cost network
+    learns, when the queue is to be ordered for another configuration than the default one."""
     say = log or (lambda *_: None)
     metric = index.metric
     proj_dim = 256 if metric == N.METRIC_HAMMING else index.row_stride
@@ -308,8 +311,11 @@ def fit(index: Index, base, train_queries, *, k=10, ef=256, n_seeds=4, bloom_bit
     model.set_on(index)
 
     # cost labels: standardised log2 hop counts of the learned search itself
-    r = index.search(train_queries, k, cost_ef or ef, bloom_bits=bloom_bits, n_seeds=n_seeds, learned=3,
-                     stop_threshold=cost_threshold, stop_min_hops=stop_min_hops, stop_interval=stop_interval)
+    copts = dict(learned=3, stop_threshold=cost_threshold, stop_min_hops=stop_min_hops, stop_interval=stop_interval)
+    if cost_search:
+        copts = {kk: v for kk, v in cost_search.items() if kk not in ("ef", "mode", "waves_per_block")}
+        copts["learned"] = copts.get("learned", 2) & 3  # the labelling run itself keeps input order
+    r = index.search(train_queries, k, cost_ef or ef, bloom_bits=bloom_bits, n_seeds=n_seeds, **copts)
     hops = np.log2(np.maximum(r.trace[:, 2], 1).astype(np.float64))
     mean, std = hops.mean(), max(hops.std(), 1e-6)
     target = ((hops - mean) / std).astype(np.float32)
