@@ -45,6 +45,11 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
 
 
+# The stop network is consulted every STOP_INTERVAL hops and trained at that
+# interval (two of its features count events since the last consult).
+STOP_INTERVAL = 8
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -284,7 +289,7 @@ def main():
             model.set_on(idx)
             log(f"learned model loaded from {args.model}")
         else:
-            model = learned.fit(idx, base, train, k=args.k, ef=384, stop_min_hops=16, stop_interval=4,
+            model = learned.fit(idx, base, train, k=args.k, ef=384, stop_min_hops=16, stop_interval=STOP_INTERVAL,
                                 max_evals=192, entry_epochs=20, stop_epochs=4, cost_threshold=3.0, log=log)
             log(f"learned model trained in {time.time() - t0:.1f}s")
             if args.model:
@@ -292,9 +297,9 @@ def main():
         # result_cap: the result set holds 16 or 32 (>= k) instead of the queue size, and the search
         # ends by the stop network or the hop cap, not by the reference's test against the worst
         # of ef results (see include/song.h) -- which lets a short queue search long enough
-        grid = {"ef": [64, 96, 128, 160, 192, 256], "learned": [2], "stop_threshold": [1.5, 1.75, 2.0, 2.5],
-                "stop_confirm": [1, 2, 3], "stop_max_hops": [600], "result_cap": [16, 32], "stop_min_hops": [16],
-                "stop_interval": [4], "mode": [1], "waves_per_block": [4]}
+        grid = {"ef": [64, 96, 128], "learned": [2], "stop_threshold": [1.25, 1.5, 1.75, 2.0],
+                "stop_confirm": [1, 2, 3], "stop_max_hops": [600], "result_cap": [16], "stop_min_hops": [16],
+                "stop_interval": [STOP_INTERVAL], "mode": [1], "waves_per_block": [4]}
         given = json.loads(args.learned_config) if args.learned_config else stored.get("learned")
         if given:
             grid = {kk: [v] for kk, v in given.items()}
