@@ -1,0 +1,76 @@
+"""The benchmark workload at full size (1,000,000 x 128 integer GMM, out-degree
+32, 10,000 queries, the graph bench.py builds): properties that do not depend
+on size, checked where the timed numbers come from.
+
+* A sample of queries against the oracle, bit for bit, at the tuned plain
+  queue (352): ids, distances and the reference's trace counters.
+* The persistent kernel against the plain one on all 10,000 queries.
+* The learned search's result cap with a stop that never fires, against the
+  oracle's restatement of that option.
+* Recall@10 of both tuned methods against the GPU brute force: >= 0.90.
+"""
+import importlib
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+learned = importlib.import_module("2020_song_ex_amd.learned")
+
+
+@pytest.fixture(scope="module")
+def full(song, datasets):
+    cfg = datasets.config2(n_extra=25000)  # bench.py's draw: training queries first
+    base, q = cfg["base"], cfg["queries"]
+    adj = song.nsw_build(base, block=32, n_threads=16, batch_div=64, seq_prefix=1024, exact_order=False)
+    idx = song.Index(base, adj)
+    gt, _, _ = idx.bruteforce(q, 10)
+    yield base, q, adj, idx, gt, cfg["extra"][:15000]
+    idx.close()
+
+
+def recall(ids, gt):
+    return float(np.mean([len(set(a) & set(b)) / 10 for a, b in zip(ids, gt)]))
+
+
+def test_sample_matches_oracle(full, oracle):
+    base, q, adj, idx, gt, _ = full
+    sample = q[::50]  # 200 queries
+    r = idx.search(sample, 10, 352)
+    # 16 lanes per 128-float row; integer data sums exactly in any order
+    oi, od, ot = oracle.search(base, adj, sample, 10, 352, block=32, bloom_bits=1 << 14, group=16)
+    np.testing.assert_array_equal(r.ids, oi)
+    np.testing.assert_array_equal(r.dists.view(np.uint32), od.view(np.uint32))
+    np.testing.assert_array_equal(r.trace, ot)
+
+
+def test_persistent_equals_plain(full):
+    base, q, adj, idx, gt, _ = full
+    a = idx.search(q, 10, 352, mode=0, waves_per_block=1)
+    b = idx.search(q, 10, 352, mode=1)
+    np.testing.assert_array_equal(a.ids, b.ids)
+    np.testing.assert_array_equal(a.trace, b.trace)
+    assert recall(a.ids, gt) >= 0.90
+
+
+def test_learned_search(full, oracle):
+    base, q, adj, idx, gt, extra = full
+    # the model bench.py trains (same queries, same settings; training is deterministic)
+    learned.fit(idx, base, extra, k=10, ef=384, stop_min_hops=16, stop_interval=8, max_evals=192,
+                entry_epochs=20, stop_epochs=4, cost_threshold=3.0)
+    # the tuned configuration reaches the benchmark's recall
+    tuned = dict(learned=2, stop_threshold=1.75, stop_confirm=2, stop_max_hops=600, result_cap=16,
+                 stop_min_hops=16, stop_interval=8, mode=1, waves_per_block=4)
+    r = idx.search(q, 10, 64, **tuned)
+    assert recall(r.ids, gt) >= 0.90
+    assert (r.trace[:, 2] < 436).mean() > 0.5  # fewer expansions than the plain search's mean
+    # result cap with a stop that never fires: the oracle's capped search with that result set
+    sample = q[::100]
+    s = idx.search(sample, 10, 64, learned=2, stop_threshold=1e30, stop_min_hops=16, stop_interval=8,
+                   stop_max_hops=150, result_cap=16)
+    oi, od, ot = oracle.search(base, adj, sample, 10, 64, block=32, bloom_bits=1 << 14, group=16, max_hops=150,
+                               result_cap=16)
+    np.testing.assert_array_equal(s.ids, oi)
+    np.testing.assert_array_equal(s.dists.view(np.uint32), od.view(np.uint32))
+    np.testing.assert_array_equal(s.trace, ot)
