@@ -96,8 +96,8 @@ def main():
     ap.add_argument("--train", type=int, default=15000, help="training queries for the learned search")
     ap.add_argument("--valid", type=int, default=10000, help="validation queries for the autotuner")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--serve-loads", default="0.2,0.4,0.6",
-                    help="Poisson rates as fractions of the batch QPS; the middle one is the headline")
+    ap.add_argument("--serve-loads", default="0.1,0.2,0.3",
+                    help="Poisson rates as fractions of the batch QPS (the launch-slot server saturates near a quarter of it); the middle one is the headline")
     ap.add_argument("--serve-batch", type=int, default=256)
     ap.add_argument("--serve-streams", type=int, default=16)
     ap.add_argument("--copy-calibration", type=int, default=0,
