@@ -220,6 +220,36 @@ def test_result_cap_options(song, setup):
             idx.search(q[:4], 10, 128, **bad)
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_result_cap_with_mixed_budgets(song, setup, mode):
+    """Per-query k and queue size in one launch with a result cap (the serving loop's path):
+    each query equals its uniform search -- queues above the cap without the reference's early
+    stop, queues at the cap with it."""
+    base, q, adj, idx, model = setup
+    N = song._native
+    nq = q.shape[0]
+    rng = np.random.default_rng(1)
+    ks = rng.choice([1, 10], size=nq).astype(np.int32)
+    efs = rng.choice([16, 64, 128], size=nq).astype(np.int32)
+    kw = dict(learned=2, stop_threshold=2.0, stop_min_hops=8, stop_interval=2, stop_max_hops=200, result_cap=16)
+    dq = idx.queries_to_device(q)
+    d_ks = N.DeviceBuffer(nq * 4).upload(ks)
+    d_efs = N.DeviceBuffer(nq * 4).upload(efs)
+    d_ids = N.DeviceBuffer(nq * 10 * 4)
+    d_tr = N.DeviceBuffer(nq * 48)
+    idx.search_device_mixed(dq, nq, d_ks, d_efs, 10, 128, d_ids, None, d_tr, mode=mode, **kw)
+    ids = d_ids.download(np.empty((nq, 10), np.int32))
+    tr = d_tr.download(np.empty((nq, 12), np.int32))
+    for kk in (1, 10):
+        for ee in (16, 64, 128):
+            sel = np.where((ks == kk) & (efs == ee))[0]
+            if sel.size == 0:
+                continue
+            r = idx.search(q[sel], kk, ee, **kw)
+            np.testing.assert_array_equal(ids[sel, :kk], r.ids)
+            np.testing.assert_array_equal(tr[sel], r.trace)
+
+
 def test_confirmed_stop_runs_longer(song, setup):
     """Needing two consecutive stop votes never stops earlier than one vote,
     and one confirmation is the default."""
