@@ -297,8 +297,9 @@ def main():
         # result_cap: the result set holds 16 or 32 (>= k) instead of the queue size, and the search
         # ends by the stop network or the hop cap, not by the reference's test against the worst
         # of ef results (see include/song.h) -- which lets a short queue search long enough
-        grid = {"ef": [64, 96, 128], "learned": [2], "stop_threshold": [1.25, 1.5, 1.75, 2.0],
-                "stop_confirm": [1, 2, 3], "stop_max_hops": [600], "result_cap": [16], "stop_min_hops": [16],
+        # the hop cap bounds the launch's last queries (their hops at full load set its end)
+        grid = {"ef": [64, 96], "learned": [2], "stop_threshold": [1.75, 2.0, 2.5],
+                "stop_confirm": [1, 2], "stop_max_hops": [400, 500, 600], "result_cap": [16], "stop_min_hops": [16],
                 "stop_interval": [STOP_INTERVAL], "mode": [1], "waves_per_block": [4]}
         given = json.loads(args.learned_config) if args.learned_config else stored.get("learned")
         if given:
