@@ -148,6 +148,39 @@ def config2(seed=0, n=1_000_000, nq=10_000, n_extra=0):
     return cfg
 
 
+def config3(seed=0, n=1_000_000, nq=1_000):
+    """1,000,000 x 960 values in [0, 1] around 500 centres perturbed by Beta(2, 5) noise, 1,000
+    queries; out-degree 32, L2, k=10 (the high-dimensional plain and persistent kernels)."""
+    base, q = beta_mix(n, nq, dim=960, n_centres=500, seed=seed)
+    return {"base": base, "queries": q, "block": 32, "k": 10}
+
+
+def config4(seed=0, n=1_000_000, nq=10_000, n_extra=0):
+    """1,000,000 unit vectors (dim 256, 2,000-centre vMF mixture, kappa 50) hashed to 256-bit
+    codes (uint64 [., 4]) by seeded Gaussian hyperplanes; 10,000 query codes; out-degree 32,
+    Hamming, k=10 (the Hamming path with learned entry and termination).
+    n_extra: further query codes (training / validation)."""
+    vec, qv = vmf_mix(n, nq + n_extra, dim=256, n_centres=2000, kappa=50.0, seed=seed)
+    codes, planes = hyperplane_codes(vec, seed=seed + 1)
+    del vec
+    qcodes, _ = hyperplane_codes(qv, planes=planes)
+    cfg = {"base": codes, "queries": qcodes[:nq], "block": 32, "k": 10}
+    if n_extra:
+        cfg["extra"] = qcodes[nq:]
+    return cfg
+
+
+def config5(seed=0, n=1_200_000, nq=10_000, n_extra=0):
+    """1,200,000 unit vectors (dim 200, 5,000-centre vMF mixture, kappa 30) searched by inner
+    product; out-degree 32; requests with k in {1, 10, 100} and queue budgets in {64, 256}
+    (the serving workload).  n_extra: further query draws."""
+    base, q = vmf_mix(n, nq + n_extra, dim=200, n_centres=5000, kappa=30.0, seed=seed)
+    cfg = {"base": base, "queries": q[:nq], "block": 32, "ks": (1, 10, 100), "efs": (64, 256)}
+    if n_extra:
+        cfg["extra"] = q[nq:]
+    return cfg
+
+
 # ------------------------------------------------------------------ file I/O
 
 def read_fvecs(path, limit=None, dtype="<f4"):

@@ -129,3 +129,21 @@ def test_build_rejects_bad_arguments(song):
         song.nsw_build(data, block=8, fixed_deg=7, search_deg=8)
     with pytest.raises(song.SongError):
         song.nsw_build(data, block=65)
+
+
+def test_workload_generators(datasets):
+    """The benchmark workloads' generators, at small n: shapes, value ranges and dtypes as
+    described (960-d values in [0, 1]; 256-bit codes as 4 x uint64 whose bits follow the sign
+    of the hyperplane projections; unit vectors for the inner-product workload)."""
+    c3 = datasets.config3(n=500, nq=20)
+    assert c3["base"].shape == (500, 960) and c3["queries"].shape == (20, 960)
+    assert c3["base"].min() >= 0.0 and c3["base"].max() <= 1.0
+    c4 = datasets.config4(n=500, nq=20, n_extra=30)
+    assert c4["base"].shape == (500, 4) and c4["base"].dtype == np.uint64
+    assert c4["queries"].shape == (20, 4) and c4["extra"].shape == (30, 4)
+    bits = np.unpackbits(c4["base"].view(np.uint8), axis=1)
+    assert 0.3 < bits.mean() < 0.7  # random hyperplanes split the sphere about evenly
+    c5 = datasets.config5(n=500, nq=20, n_extra=10)
+    assert c5["base"].shape == (500, 200) and c5["extra"].shape == (10, 200)
+    np.testing.assert_allclose(np.linalg.norm(c5["base"], axis=1), 1.0, rtol=1e-5)
+    assert set(c5["ks"]) == {1, 10, 100} and set(c5["efs"]) == {64, 256}
