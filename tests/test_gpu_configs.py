@@ -5,7 +5,7 @@ distributions, with the base scaled down so that the CPU graph build stays withi
   configuration the autotuner tries for the persistent kernel (queue, waves per CU, Bloom size)
   returns the oracle's search with that queue and Bloom size, and the persistent grid returns
   the plain launch.
-* 256-bit Hamming codes of a 2,000-centre vMF mixture (1,000,000 codes; 200,000 here): the oracle
+* 256-bit Hamming codes of a 2,000-centre vMF mixture, at the full 1,000,000 codes: the oracle
   at two queue sizes, persistent = plain, and the learned entry + stop trained on held-out codes.
 * Inner product on a 5,000-centre vMF mixture, dim 200 (1,200,000 vectors; 200,000 here), with
   requests of k in {1, 10, 100} and queue budgets in {64, 256}: each request of a mixed launch
@@ -79,7 +79,7 @@ def test_config3_autotuned_persistent(song, oracle, c3):
 
 @pytest.fixture(scope="module")
 def c4(song, datasets):
-    cfg = datasets.config4(n=200_000, nq=1000, n_extra=4000)
+    cfg = datasets.config4(n=1_000_000, nq=1000, n_extra=4000)
     adj = build(song, cfg["base"], "hamming")
     idx = song.Index(cfg["base"], adj, metric="hamming")
     yield cfg["base"], cfg["queries"], cfg["extra"], adj, idx
