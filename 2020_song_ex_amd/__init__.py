@@ -13,9 +13,9 @@ the C ABI of include/song.h).  This package is its Python front end:
 from . import _native
 from ._native import SongError, device_count
 from .index import (TRACE_FIELDS, Index, SearchResult, bithash_matrix, bithash_project,
-                    degrees_for_block, graph_load, graph_save, nsw_build)
+                    bithash_project_device, degrees_for_block, graph_load, graph_save, nsw_build)
 
 __all__ = [
     "SongError", "device_count", "TRACE_FIELDS", "Index", "SearchResult", "bithash_matrix",
-    "bithash_project", "degrees_for_block", "graph_load", "graph_save", "nsw_build", "_native",
+    "bithash_project", "bithash_project_device", "degrees_for_block", "graph_load", "graph_save", "nsw_build", "_native",
 ]

@@ -45,7 +45,8 @@ class SongSearchOpts(ctypes.Structure):
 EXPORTS = (
     "song_abi_version", "song_last_error", "song_device_count", "song_nsw_build",
     "song_graph_save", "song_graph_load_header", "song_graph_load", "song_bithash_matrix",
-    "song_bithash_project", "song_index_create", "song_index_destroy", "song_index_row_stride",
+    "song_bithash_project", "song_bithash_project_device", "song_index_create", "song_index_destroy",
+    "song_index_row_stride",
     "song_search", "song_search_device", "song_dev_alloc", "song_dev_free", "song_memcpy_h2d",
     "song_memcpy_d2h", "song_queries_to_device", "song_index_bruteforce",
     "song_search_device_mixed", "song_dev_copy", "song_index_set_model",
@@ -80,6 +81,7 @@ def lib():
     L.song_graph_load.argtypes = [ctypes.c_char_p, _P, _I64, _I32]
     L.song_bithash_matrix.argtypes = [_I32, ctypes.c_uint32, _P]
     L.song_bithash_project.argtypes = [_P, _I32, _P, _I64, _I32, _P]
+    L.song_bithash_project_device.argtypes = [_P, _I32, _P, _I64, _I32, _P, _P]
     L.song_index_create.argtypes = [_P, _P, _I64, _I32, _I32, _I32, _I32, ctypes.POINTER(_P)]
     L.song_index_destroy.argtypes = [_P]
     L.song_index_destroy.restype = None

@@ -973,6 +973,32 @@ int song_index_bruteforce(song_index* ix, const void* queries, int32_t nq, int32
     return SONG_OK;
 }
 
+int song_bithash_project_device(const float* d_R, int32_t dim, const float* d_vecs, int64_t n, int32_t device,
+                                uint64_t* d_codes_out, double* kernel_ms_out) {
+    if (dim < 1 || dim > 2048) return fail(SONG_ERR_ARG, "bithash on the device supports dim in [1, 2048]");
+    if (n < 0) return fail(SONG_ERR_ARG, "n must be >= 0");
+    if (!d_R || (n > 0 && (!d_vecs || !d_codes_out))) return fail(SONG_ERR_ARG, "null buffer");
+    HIP_TRY(hipSetDevice(device), "hipSetDevice");
+    hipEvent_t a, b;
+    HIP_TRY(hipEventCreate(&a), "hipEventCreate");
+    hipError_t e = hipEventCreate(&b);
+    if (e != hipSuccess) {
+        (void)hipEventDestroy(a);
+        return hip_fail(e, "hipEventCreate");
+    }
+    e = hipEventRecord(a, 0);
+    if (e == hipSuccess) e = song::launch_bithash(d_R, dim, d_vecs, n, d_codes_out, 0);
+    if (e == hipSuccess) e = hipEventRecord(b, 0);
+    if (e == hipSuccess) e = hipEventSynchronize(b);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (e != hipSuccess) return hip_fail(e, "bithash projection");
+    if (kernel_ms_out) *kernel_ms_out = ms;
+    return SONG_OK;
+}
+
 int song_dev_alloc(int32_t device, int64_t bytes, void** out) {
     if (!out || bytes < 0) return fail(SONG_ERR_ARG, "bad dev_alloc arguments");
     *out = nullptr;

@@ -138,4 +138,8 @@ hipError_t launch_search_ip(const SearchArgs& a, int S, int G, int qch, const La
 hipError_t launch_search_hamming(const SearchArgs& a, int S, int G, int qch, const LaunchShape& shape,
                                  hipStream_t st);
 
+// BitHash codes of n vectors (uint64 [n x 4]) from R [256 x dim], dim <= 2048,
+// all in device memory: the sequential float dot products of song_bithash_project.
+hipError_t launch_bithash(const float* R, int dim, const float* X, int64_t n, uint64_t* codes, hipStream_t st);
+
 }  // namespace song

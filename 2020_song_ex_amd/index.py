@@ -90,6 +90,20 @@ def bithash_project(R, vecs, n_threads=1) -> np.ndarray:
     return out
 
 
+def bithash_project_device(R, vecs, device=0):
+    """The same codes computed on the GPU (song_bithash_project_device): (uint64 [n, 4], kernel ms)."""
+    R = np.ascontiguousarray(R, dtype=np.float32)
+    v = np.ascontiguousarray(vecs, dtype=np.float32)
+    n, dim = v.shape
+    d_R = N.DeviceBuffer(R.nbytes, device).upload(R)
+    d_v = N.DeviceBuffer(max(v.nbytes, 4), device).upload(v)
+    d_c = N.DeviceBuffer(max(n * 32, 8), device)
+    ms = ctypes.c_double(0.0)
+    N.check(N.lib().song_bithash_project_device(d_R.ptr, dim, d_v.ptr, n, device, d_c.ptr, ctypes.byref(ms)),
+            "song_bithash_project_device")
+    return d_c.download(np.empty((n, 4), np.uint64)), ms.value
+
+
 @dataclass
 class SearchResult:
     ids: np.ndarray

@@ -189,6 +189,15 @@ int song_graph_load(const char* path, int32_t* adj_out, int64_t n, int32_t block
 int song_bithash_matrix(int32_t dim, uint32_t seed, float* out);
 int song_bithash_project(const float* R, int32_t dim, const float* vecs, int64_t n, int32_t n_threads,
                          uint64_t* codes_out);
+/* The same projection on the GPU, the same codes bit for bit (each dot product
+ * summed in order in float, each product rounded on its own; no fused or
+ * reordered sums).  Replaces BitHash::project / project_all (bitHash.h:32-55)
+ * on the Hamming query path.  Every buffer is in device memory of `device`:
+ * d_R [256 x dim] (song_bithash_matrix's layout), d_vecs [n x dim] f32,
+ * d_codes_out [n x 4] uint64.  dim in [1, 2048].  Synchronous; kernel_ms_out
+ * (nullable) gets the kernel's time. */
+int song_bithash_project_device(const float* d_R, int32_t dim, const float* d_vecs, int64_t n, int32_t device,
+                                uint64_t* d_codes_out, double* kernel_ms_out);
 
 /* ---------------------------------------------------------------- index */
 

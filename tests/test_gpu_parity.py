@@ -298,3 +298,26 @@ def test_gpu_bruteforce(song, oracle, datasets, metric, k):
     else:
         np.testing.assert_array_equal(ids, oi)
         np.testing.assert_array_equal(d, od.astype(np.float32))
+
+
+@pytest.mark.parametrize("dim", [1, 7, 64, 200, 256, 960])
+def test_bithash_on_gpu(song, dim):
+    """The GPU projection gives the CPU projection's codes bit for bit (the CPU one equals the
+    reference's own BitHash, tests/test_build.py), for any dim and a partial last wave, with a
+    zero vector (every bit 0), a vector and its negation, tiny values, and a vector made
+    orthogonal to row 0 of R so that its first sum sits at rounding level."""
+    rng = np.random.default_rng(dim)
+    R = song.bithash_matrix(dim, 42)
+    v = rng.normal(size=(1003, dim)).astype(np.float32)
+    v[5] = 0.0
+    v[7] = -v[8]
+    v[6] *= np.float32(1e-30)
+    r0 = R[0].astype(np.float64)
+    x = v[9].astype(np.float64)
+    v[9] = (x - (x @ r0) / (r0 @ r0) * r0).astype(np.float32)
+    cpu = song.bithash_project(R, v)
+    gpu, ms = song.bithash_project_device(R, v)
+    np.testing.assert_array_equal(gpu, cpu)
+    assert (gpu[5] == 0).all() and ms > 0
+    with pytest.raises(song.SongError, match="dim"):
+        song.bithash_project_device(np.zeros((256, 4096), np.float32), np.zeros((2, 4096), np.float32))
