@@ -22,7 +22,9 @@ both are in the JSON line with their ratio.  The line also carries the
 roofline of the search kernel behind `value` (algorithmic bytes from the run's
 own trace counters / HIP-event kernel time), Poisson-load serving latency (p50 /
 p99) and the reference's CPU search (Graph::search, graph.h:33-115, compiled
-from its sources into oracle/_ref) timed on this host on a bounded sample.
+from its sources into oracle/_ref) timed on this host on a bounded sample with
+one thread (the reference's own setting) and on all queries split over the
+process's cores (cpu_baseline.all_cores), with the CPU model.
 """
 import os
 
@@ -59,6 +61,17 @@ def recall_at(ids, gt, k):
     for a, g in zip(ids[:, :k], gt[:, :k]):
         hits += len(set(a.tolist()) & set(g.tolist()))
     return hits / (ids.shape[0] * k)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def algorithmic_bytes(trace, dim, block, k):
@@ -381,7 +394,13 @@ def main():
         cpu = {"value": round(sample / (cms / 1e3), 1), "unit": "queries/s", "cores": 1, "kind": kind,
                "sample": f"{sample} queries of the same workload, queue {ef}, k {args.k}, 1 thread "
                          f"(Graph::search, graph.h:33-115)",
-               "recall_at_10": round(recall_at(cids, gt[:sample], 10), 4)}
+               "recall_at_10": round(recall_at(cids, gt[:sample], 10), 4),
+               "cpu_model": cpu_model()}
+        if kind == "reference":
+            # the same search with the queries split over the host's cores (this process's share)
+            _, _, ams, _ = oracle.ref_graph_search(base, adj, queries, args.k, ef, n_threads=threads)
+            cpu["all_cores"] = {"value": round(nq / (ams / 1e3), 1), "cores": threads,
+                                "sample": f"all {nq} queries split over {threads} threads"}
         log(f"cpu baseline {cpu}")
 
     use_learned = (learned_out is not None and "qps" in learned_out
