@@ -175,15 +175,17 @@ def main():
         return ids, tr, tm
 
     def timed(k, ef, **kw):
+        """(queries/s over the timed steps' wall clock, ms per step, mean and population std of the
+        steps' HIP-event kernel times)"""
         for _ in range(args.warmup):
             idx.search_device(dq, nq, k, ef, d_ids, None, None, **kw)
-        kernel_ms = 0.0
+        kms = []
         t0 = time.perf_counter()
         for _ in range(args.steps):
             tm = idx.search_device(dq, nq, k, ef, d_ids, None, None, **kw)
-            kernel_ms += tm["kernel_ms"]
+            kms.append(tm["kernel_ms"])
         wall = time.perf_counter() - t0
-        return nq * args.steps / wall, wall * 1e3 / args.steps, kernel_ms / args.steps
+        return nq * args.steps / wall, wall * 1e3 / args.steps, float(np.mean(kms)), float(np.std(kms))
 
     sweep = []
     efs = (32, 64, 128, 256, 512)
@@ -242,7 +244,7 @@ def main():
                 break
     ids, trace, _ = run(args.k, ef, **plain_params)
     plain_rec = recall_at(ids, gt, 10)
-    plain_qps, plain_ms_step, plain_kernel_ms = timed(args.k, ef, **plain_params)
+    plain_qps, plain_ms_step, plain_kernel_ms, plain_kernel_std = timed(args.k, ef, **plain_params)
     log(f"plain: queue {ef} {plain_params}, recall@10 {plain_rec:.4f}, {plain_qps:.0f} queries/s")
 
     def slots_for(q):
@@ -286,7 +288,7 @@ def main():
     log(f"plain through host buffers (PCIe included): {pcie_qps:.0f} queries/s")
 
     # ---------------- persistent kernel alone (no learning), for the record
-    pers_qps, _, pers_kernel_ms = timed(args.k, ef, mode=1)
+    pers_qps, _, pers_kernel_ms, _ = timed(args.k, ef, mode=1)
     log(f"persistent (no learning): queue {ef}, {pers_qps:.0f} queries/s")
 
     # ---------------- learned + persistent
@@ -331,13 +333,14 @@ def main():
             lef = p.pop("ef")
             ids, ltr, _ = run(args.k, lef, **p)
             lrec = recall_at(ids, gt, 10)
-            lqps, lms_step, lkernel_ms = timed(args.k, lef, **p)
+            lqps, lms_step, lkernel_ms, lkernel_std = timed(args.k, lef, **p)
             best_search = (lef, p)
             lname = f"persistent_kernel<0,{slots_for(lef)},{lanes},{chunks},1>"
             lkey = f"{lname} {args.n}x{dim} nq={nq} " + " ".join(f"{kk}={v}" for kk, v in sorted({"ef": lef, **p}.items()))
             learned_roofline = roofline_of(ltr, lkernel_ms, lname, lkey)
             learned_out = {"params": {"ef": lef, **p}, "recall_at_10": round(lrec, 4), "qps": round(lqps, 1),
                            "ms_per_step": round(lms_step, 4), "kernel_ms": round(lkernel_ms, 4),
+                           "kernel_ms_std": round(lkernel_std, 4), "qps_kernel": round(nq / (lkernel_ms / 1e3), 1),
                            "expanded_mean": float(ltr[:, 2].mean()),
                            "learned_stops": float((ltr[:, 11] == 2).mean()),
                            "hop_capped": float((ltr[:, 11] == 3).mean()),
@@ -418,7 +421,9 @@ def main():
                                f"{nq} queries, NSW out-degree {block}, L2, k {args.k}",
                    "method": "learned+persistent" if use_learned else "plain",
                    "plain": {"queue": ef, **plain_params, "recall_at_10": round(plain_rec, 4),
-                             "qps": round(plain_qps, 1), "kernel_ms": round(plain_kernel_ms, 4)},
+                             "qps": round(plain_qps, 1), "kernel_ms": round(plain_kernel_ms, 4),
+                             "kernel_ms_std": round(plain_kernel_std, 4),
+                             "qps_kernel": round(nq / (plain_kernel_ms / 1e3), 1)},
                    "plain_host_buffers_qps": round(pcie_qps, 1),
                    "persistent_no_learning": {"queue": ef, "qps": round(pers_qps, 1),
                                               "kernel_ms": round(pers_kernel_ms, 4)},
