@@ -329,7 +329,8 @@ int main(int argc, char** argv) {
             auto th0 = clk::now();
             hkg.push_back(song::build_hashed_graph(base.data(), o.n, dim, bc));
             auto th1 = clk::now();
-            qhash = hkg[0].bh.project_all(query.data(), o.nq, o.threads);
+            // queries hashed on the GPU (the reference's BitHash::project_all codes, bit for bit)
+            qhash = hkg[0].bh.project_all_gpu(query.data(), o.nq);
             auto th2 = clk::now();
             hkg[0].upload_to_gpu();
             auto th3 = clk::now();

@@ -201,6 +201,30 @@ struct BitHash {
         check(song_bithash_project(R.data(), dim, data, n, n_threads, (uint64_t*)out.data()), "song_bithash_project");
         return out;
     }
+    // project_all on GPU `device` (song_bithash_project_device): the same codes bit for bit,
+    // host buffers in and out.
+    std::vector<uint32_t> project_all_gpu(const float* data, int n, int device = 0) const {
+        std::vector<uint32_t> out((size_t)n * HASH_WORDS);
+        void *d_R = nullptr, *d_x = nullptr, *d_c = nullptr;
+        auto release = [&] {
+            song_dev_free(d_R);
+            song_dev_free(d_x);
+            song_dev_free(d_c);
+        };
+        int rc = song_dev_alloc(device, (int64_t)R.size() * 4, &d_R);
+        if (rc == SONG_OK) rc = song_dev_alloc(device, (int64_t)n * dim * 4, &d_x);
+        if (rc == SONG_OK) rc = song_dev_alloc(device, (int64_t)n * 32, &d_c);
+        if (rc == SONG_OK) rc = song_memcpy_h2d(d_R, R.data(), (int64_t)R.size() * 4);
+        if (rc == SONG_OK) rc = song_memcpy_h2d(d_x, data, (int64_t)n * dim * 4);
+        if (rc == SONG_OK)
+            rc = song_bithash_project_device((const float*)d_R, dim, (const float*)d_x, n, device, (uint64_t*)d_c,
+                                             nullptr);
+        if (rc == SONG_OK) rc = song_memcpy_d2h(out.data(), d_c, (int64_t)n * 32);
+        const std::string err = rc == SONG_OK ? "" : song_last_error();
+        release();
+        if (rc != SONG_OK) throw Error(std::string("song_bithash_project_device: ") + err);
+        return out;
+    }
 };
 
 // HashKernelGraph (hashKernelGraph.h:22-126)

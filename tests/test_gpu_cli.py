@@ -74,3 +74,21 @@ def test_raw_ids_equal_the_library_search(song, compare_run):
     finally:
         idx.close()
     np.testing.assert_array_equal(got, r.ids)
+
+
+def test_hash_ids_equal_the_library_search(song, compare_run):
+    """bench_gpu's hash method (queries hashed on the GPU) is the library's Hamming search over
+    the CPU BitHash codes (seed 42, bitHash.h:22) on the reference-built code graph."""
+    base, q, out = compare_run
+    runs = out / "runs" / "repro_gpu"
+    got = np.array([[int(t) for t in line.split()] for line in open(runs / "repro_gpu_hash_pq0064.ids")])
+    R = song.bithash_matrix(base.shape[1], 42)
+    codes = song.bithash_project(R, base)
+    qcodes = song.bithash_project(R, q)
+    adj = song.nsw_build(codes, metric="hamming", block=32, n_threads=4)
+    idx = song.Index(codes, adj, metric="hamming")
+    try:
+        r = idx.search(qcodes, 10, 64, batch=32)
+    finally:
+        idx.close()
+    np.testing.assert_array_equal(got, r.ids)
