@@ -72,3 +72,32 @@ def test_driver_reads_reference_shaped_summary(tmp_path):
         f.write(",".join(heads) + "\n" + ",".join(row[h] for h in heads) + "\n")
     rows = rc.load_sweep_rows(p, 10)
     assert rows[0]["qps"] == 112736 and rows[0]["recall"] == 0.8838 and rows[0]["pq"] == 50
+
+
+def test_reference_vectors(song, oracle):
+    """Outputs of the reference's own build_graph, Graph::search and BitHash, compiled from its
+    sources (tests/golden/make_ref_vectors.py), as a committed fixture: the product's NSW build
+    and BitHash equal them, and so do the oracle's Graph::search port and, on float data with
+    distinct distances, its GPU-algorithm re-execution under the exact-visited,
+    unbounded-frontier and sequential-sum knobs.  Holds where the reference sources are absent."""
+    d = np.load(os.path.join(GOLDEN, "reference_vectors.npz"))
+    for tag in ("int", "float"):
+        base, q, adj = d[f"{tag}_base"], d[f"{tag}_queries"], d[f"{tag}_adj"]
+        np.testing.assert_array_equal(song.nsw_build(base, block=32, exact_order=True), adj)
+        for ef in (10, 50):
+            rid, rtr = d[f"{tag}_ids_ef{ef}"], d[f"{tag}_trace_ef{ef}"]
+            pid, ptr = oracle.cpu_search(base, adj, q, 10, ef, block=32)
+            np.testing.assert_array_equal(pid, rid)
+            np.testing.assert_array_equal(ptr, rtr)
+            if tag == "float":
+                oid, _, otr = oracle.search(base, adj, q, 10, ef, block=32, ref_order=True, exact_visited=True,
+                                            unbounded=True)
+                cnt = (rid >= 0).sum(1)
+                for i in range(q.shape[0]):  # Graph::search returns worst-first
+                    np.testing.assert_array_equal(oid[i, :cnt[i]], rid[i, :cnt[i]][::-1])
+                np.testing.assert_array_equal(otr, rtr)
+    R = song.bithash_matrix(24, 42)
+    np.testing.assert_array_equal(R, d["bithash_matrix"])
+    v = d["bithash_vectors"]
+    codes = song.bithash_project(R, v)
+    np.testing.assert_array_equal(codes.view(np.uint32).reshape(v.shape[0], 8), d["bithash_codes"])
