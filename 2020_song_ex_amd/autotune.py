@@ -60,6 +60,22 @@ def measure(index, dq, nq, k, params, gt, d_ids, d_tr, repeats=3):
     return recall_at(ids, gt, k), best, float(tr[:, 2].mean())
 
 
+def candidates(grid: dict):
+    """The grid's configurations in the order they are tried (product over the keys in order)."""
+    names = list(grid)
+    return [dict(zip(names, values)) for values in itertools.product(*[grid[n] for n in names])]
+
+
+def pick(tried, target_recall):
+    """The candidate with the highest throughput among those whose recall meets the target; on equal
+    throughput the earlier one.  None if no candidate meets it."""
+    best = None
+    for c in tried:
+        if c.recall >= target_recall and (best is None or c.qps > best.qps):
+            best = c
+    return best
+
+
 def tune(index, queries, gt, k, grid: dict, target_recall, repeats=3, log=None) -> TuneResult:
     """Grid search over `grid` (name -> list of values, including "ef")."""
     say = log or (lambda *_: None)
@@ -67,14 +83,11 @@ def tune(index, queries, gt, k, grid: dict, target_recall, repeats=3, log=None) 
     dq = index.queries_to_device(queries)
     d_ids = N.DeviceBuffer(nq * k * 4)
     d_tr = N.DeviceBuffer(nq * 48)
-    names = list(grid)
     res = TuneResult(None)
-    for values in itertools.product(*[grid[n] for n in names]):
-        params = dict(zip(names, values))
+    for params in candidates(grid):
         rec, ms, exp = measure(index, dq, nq, k, params, gt, d_ids, d_tr, repeats)
         c = Candidate(params, rec, ms, nq / (ms / 1e3), exp)
         res.tried.append(c)
         say(f"tune {params} recall {rec:.4f} kernel {ms:.3f} ms qps {c.qps:.0f}")
-        if rec >= target_recall and (res.best is None or c.qps > res.best.qps):
-            res.best = c
+    res.best = pick(res.tried, target_recall)
     return res

@@ -64,3 +64,22 @@ def test_null_arguments(song):
 
 def test_device_count_does_not_fail(song):
     assert song.device_count() >= 0
+
+
+def test_autotuner_pick_is_deterministic():
+    """The tuner tries the grid in product order and picks, among candidates whose recall meets
+    the target, the highest throughput -- the earlier one on a tie; none when none meets it."""
+    import importlib
+    at = importlib.import_module("2020_song_ex_amd.autotune")
+    grid = {"ef": [64, 128], "mode": [0, 1], "bloom_bits": [1 << 13]}
+    order = at.candidates(grid)
+    assert order == [{"ef": 64, "mode": 0, "bloom_bits": 8192}, {"ef": 64, "mode": 1, "bloom_bits": 8192},
+                     {"ef": 128, "mode": 0, "bloom_bits": 8192}, {"ef": 128, "mode": 1, "bloom_bits": 8192}]
+    C = at.Candidate
+    tried = [C(order[0], recall=0.85, qps=9e6), C(order[1], recall=0.91, qps=5e6),
+             C(order[2], recall=0.95, qps=5e6), C(order[3], recall=0.97, qps=3e6)]
+    assert at.pick(tried, 0.90) is tried[1]  # fastest above the target; the tie keeps the earlier
+    assert at.pick(tried, 0.80) is tried[0]
+    assert at.pick(tried, 0.96) is tried[3]
+    assert at.pick(tried, 0.99) is None
+    assert at.pick(list(reversed(tried)), 0.90) is tried[2]  # order decides ties
