@@ -338,6 +338,11 @@ int launch_mixed_on_stream(song_index* ix, const void* d_q, int nq, const int32_
 
 int64_t plan_ws_words(int64_t nq) { return plan_words(nq, 64); }
 
+int check_search_opts(const song_index* ix, int32_t k_max, int32_t ef_max, const song_search_opts* opts) {
+    Plan p;
+    return make_plan(ix, 1, k_max, ef_max, opts, &p);
+}
+
 int launch_resident_on_stream(song_index* ix, const void* d_q, int ring, const int32_t* d_ks, const int32_t* d_efs,
                               int k_stride, int ef_max, const song_search_opts* opts, int32_t* d_ids, float* d_dists,
                               const void* d_queue, hipStream_t stream) {

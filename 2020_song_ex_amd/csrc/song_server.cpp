@@ -130,7 +130,7 @@ double ms_between(song::clk::time_point a, song::clk::time_point b) {
 }
 
 // Launch one slot's batch (called by the dispatcher without the lock held).
-hipError_t launch_slot(song_server* sv, song::Slot& s) {
+hipError_t launch_slot(song_server* sv, song::Slot& s, std::string* why) {
     const int nb = (int)s.reqs.size();
     int kmax = 1, efmax = 1;
     for (int i = 0; i < nb; i++) {
@@ -148,7 +148,10 @@ hipError_t launch_slot(song_server* sv, song::Slot& s) {
     if (e != hipSuccess) return e;
     int rc = song::launch_mixed_on_stream(sv->ix, s.d_q, nb, s.d_ks, s.d_efs, kst, efmax, &sv->opts.search, s.d_ids,
                                           s.d_dists, s.d_counter, s.d_plan, s.stream);
-    if (rc != SONG_OK) return hipErrorInvalidValue;
+    if (rc != SONG_OK) {
+        *why = song_last_error();
+        return hipErrorInvalidValue;
+    }
     e = hipMemcpyAsync(s.h_ids, s.d_ids, (size_t)nb * kst * 4, hipMemcpyDeviceToHost, s.stream);
     if (e == hipSuccess) e = hipMemcpyAsync(s.h_dists, s.d_dists, (size_t)nb * kst * 4, hipMemcpyDeviceToHost, s.stream);
     if (e == hipSuccess) e = hipEventRecord(s.done, s.stream);
@@ -206,9 +209,10 @@ void dispatcher_loop(song_server* sv) {
             s.busy = true;
             sv->stats.in_flight += (int64_t)s.reqs.size();
             lock.unlock();
-            hipError_t e = launch_slot(sv, s);
+            std::string why;
+            hipError_t e = launch_slot(sv, s, &why);
             lock.lock();
-            if (e != hipSuccess) sv->error = std::string("launch: ") + hipGetErrorString(e);
+            if (e != hipSuccess) sv->error = "launch: " + (why.empty() ? std::string(hipGetErrorString(e)) : why);
             launched = true;
         }
         bool any_busy = false;
@@ -423,6 +427,8 @@ int song_server_create(song_index* ix, const song_server_opts* o, song_server** 
     int32_t stride = 0;
     int rc = song_index_row_stride(ix, &stride);
     if (rc) return rc;
+    // refuse options every launch would refuse (e.g. a learned search with k_max > 64)
+    if ((rc = song::check_search_opts(ix, opts.k_max, opts.ef_max, &opts.search)) != SONG_OK) return rc;
     int metric = 0, device = 0;
     song::index_info(ix, &metric, &device);
     song_server* sv = new song_server();

@@ -18,6 +18,9 @@ int launch_mixed_on_stream(song_index* ix, const void* d_q, int nq, const int32_
                            int k_stride, int ef_max, const song_search_opts* opts, int32_t* d_ids, float* d_dists,
                            int32_t* d_counter, int32_t* d_plan_ws, hipStream_t stream);
 int64_t plan_ws_words(int64_t nq);
+// The checks a launch of these options makes, for requests up to k_max / ef_max
+// (learned searches need k <= 64, ...): SONG_OK or the launch's error.
+int check_search_opts(const song_index* ix, int32_t k_max, int32_t ef_max, const song_search_opts* opts);
 // The resident serving grid over a request ring (see ResidentQueue); d_queue
 // is the device copy of the ring descriptor.  No synchronisation.
 int launch_resident_on_stream(song_index* ix, const void* d_q, int ring, const int32_t* d_ks, const int32_t* d_efs,

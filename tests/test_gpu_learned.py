@@ -250,6 +250,21 @@ def test_result_cap_with_mixed_budgets(song, setup, mode):
             np.testing.assert_array_equal(tr[sel], r.trace)
 
 
+def test_learned_server_checks_k(song, setup):
+    """The learned search serves k <= 64: a server for larger k is refused when it is made, not at
+    its first launch."""
+    serving = importlib.import_module("2020_song_ex_amd.serving")
+    base, q, adj, idx, model = setup
+    with pytest.raises(song.SongError, match="k <= 64"):
+        serving.Server(idx, k_max=100, learned=2, stop_threshold=2.0)
+    sv = serving.Server(idx, k_max=64, ef_max=128, learned=2, stop_threshold=2.0, stop_min_hops=8)
+    try:
+        t = sv.submit(q[0], 10, 64)
+        assert sv.wait(t, 10) is not None
+    finally:
+        sv.close()
+
+
 def test_confirmed_stop_runs_longer(song, setup):
     """Needing two consecutive stop votes never stops earlier than one vote,
     and one confirmation is the default."""

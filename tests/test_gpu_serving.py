@@ -80,6 +80,11 @@ def test_bad_requests_are_errors(song, idx):
         sv.close()
     with pytest.raises(song.SongError):
         serving.Server(index, memory_budget=16)
+    # search options every launch would refuse are refused when the server is made
+    with pytest.raises(song.SongError, match="learned search needs a model"):
+        serving.Server(index, k_max=10, learned=2, stop_threshold=2.0)
+    with pytest.raises(song.SongError, match="result_cap"):
+        serving.Server(index, k_max=10, ef_max=64, result_cap=-1)
 
 
 # ---------------------------------------------------------------- resident kernel
