@@ -318,9 +318,10 @@ void index_info(const song_index* ix, int* metric, int* device) {
 
 int launch_mixed_on_stream(song_index* ix, const void* d_q, int nq, const int32_t* d_ks, const int32_t* d_efs,
                            int k_stride, int ef_max, const song_search_opts* opts, int32_t* d_ids, float* d_dists,
-                           int32_t* d_counter, int32_t* d_plan_ws, hipStream_t stream) {
+                           int32_t* d_counter, int32_t* d_plan_ws, hipStream_t stream, int k_limit) {
     Plan p;
-    int rc = make_plan(ix, nq, k_stride, ef_max, opts, &p);
+    if (k_limit <= 0 || k_limit > k_stride) k_limit = k_stride;
+    int rc = make_plan(ix, nq, k_limit, ef_max, opts, &p);
     if (rc) return rc;
     if (nq == 0) return SONG_OK;
     SearchArgs a = make_args(ix, p, d_q, nq, k_stride, d_ids, d_dists, nullptr);

@@ -86,6 +86,10 @@ struct song_server {
     std::atomic<int64_t> next_ticket{1};
     int64_t inflight_bytes = 0;
     song_server_stats stats{};
+    // plain fallback: with a learned search and k_max above what it serves,
+    // requests with k > fallback_k run the plain search (plain_opts)
+    int32_t fallback_k = 0;
+    song_search_opts plain_opts{};
     bool stop = false;
     std::string error;
     std::thread dispatcher;
@@ -131,23 +135,40 @@ double ms_between(song::clk::time_point a, song::clk::time_point b) {
 
 // Launch one slot's batch (called by the dispatcher without the lock held).
 hipError_t launch_slot(song_server* sv, song::Slot& s, std::string* why) {
+    // with the plain fallback, the requests beyond the learned search's k go
+    // last and run as a second launch on the same stream (rows stay in
+    // s.reqs order, which the retirement reads)
+    if (sv->fallback_k > 0)
+        std::stable_partition(s.reqs.begin(), s.reqs.end(),
+                              [&](const song::Request* r) { return r->k <= sv->fallback_k; });
     const int nb = (int)s.reqs.size();
-    int kmax = 1, efmax = 1;
+    int n1 = nb;  // requests of the configured search; the rest fall back
+    int efmax1 = 1, efmax2 = 1;
     for (int i = 0; i < nb; i++) {
         song::Request* r = s.reqs[i];
         std::memcpy(s.h_q + (size_t)i * sv->row_bytes, r->query.data(), sv->row_bytes);
         s.h_ks[i] = r->k;
         s.h_efs[i] = r->ef;
-        kmax = std::max(kmax, r->k);
-        efmax = std::max(efmax, std::max(r->ef, r->k));
+        const bool plain = sv->fallback_k > 0 && r->k > sv->fallback_k;
+        if (plain && n1 == nb) n1 = i;
+        int& efm = plain ? efmax2 : efmax1;
+        efm = std::max(efm, std::max(r->ef, r->k));
     }
     const int kst = sv->opts.k_max;
     hipError_t e = hipMemcpyAsync(s.d_q, s.h_q, (size_t)nb * sv->row_bytes, hipMemcpyHostToDevice, s.stream);
     if (e == hipSuccess) e = hipMemcpyAsync(s.d_ks, s.h_ks, (size_t)nb * 4, hipMemcpyHostToDevice, s.stream);
     if (e == hipSuccess) e = hipMemcpyAsync(s.d_efs, s.h_efs, (size_t)nb * 4, hipMemcpyHostToDevice, s.stream);
     if (e != hipSuccess) return e;
-    int rc = song::launch_mixed_on_stream(sv->ix, s.d_q, nb, s.d_ks, s.d_efs, kst, efmax, &sv->opts.search, s.d_ids,
-                                          s.d_dists, s.d_counter, s.d_plan, s.stream);
+    int rc = SONG_OK;
+    if (n1 > 0)
+        rc = song::launch_mixed_on_stream(sv->ix, s.d_q, n1, s.d_ks, s.d_efs, kst, efmax1, &sv->opts.search, s.d_ids,
+                                          s.d_dists, s.d_counter, s.d_plan, s.stream,
+                                          sv->fallback_k > 0 ? sv->fallback_k : kst);
+    if (rc == SONG_OK && n1 < nb)
+        rc = song::launch_mixed_on_stream(sv->ix, (const char*)s.d_q + (size_t)n1 * sv->row_bytes, nb - n1,
+                                          s.d_ks + n1, s.d_efs + n1, kst, efmax2, &sv->plain_opts,
+                                          s.d_ids + (size_t)n1 * kst, s.d_dists + (size_t)n1 * kst, s.d_counter,
+                                          nullptr, s.stream);
     if (rc != SONG_OK) {
         *why = song_last_error();
         return hipErrorInvalidValue;
@@ -155,7 +176,6 @@ hipError_t launch_slot(song_server* sv, song::Slot& s, std::string* why) {
     e = hipMemcpyAsync(s.h_ids, s.d_ids, (size_t)nb * kst * 4, hipMemcpyDeviceToHost, s.stream);
     if (e == hipSuccess) e = hipMemcpyAsync(s.h_dists, s.d_dists, (size_t)nb * kst * 4, hipMemcpyDeviceToHost, s.stream);
     if (e == hipSuccess) e = hipEventRecord(s.done, s.stream);
-    (void)kmax;
     return e;
 }
 
@@ -427,13 +447,32 @@ int song_server_create(song_index* ix, const song_server_opts* o, song_server** 
     int32_t stride = 0;
     int rc = song_index_row_stride(ix, &stride);
     if (rc) return rc;
-    // refuse options every launch would refuse (e.g. a learned search with k_max > 64)
-    if ((rc = song::check_search_opts(ix, opts.k_max, opts.ef_max, &opts.search)) != SONG_OK) return rc;
+    // The learned search serves k <= LEARNED_K_MAX; with a larger k_max the
+    // requests above it fall back to the plain search (the same options
+    // without the learned ones).  Options every launch would refuse are
+    // refused here.
+    constexpr int32_t LEARNED_K_MAX = 64;
+    int32_t fallback_k = 0;
+    song_search_opts plain_opts = opts.search;
+    if (opts.search.learned && opts.k_max > LEARNED_K_MAX && !opts.resident) {
+        fallback_k = LEARNED_K_MAX;
+        plain_opts.learned = 0;
+        plain_opts.stop_threshold = 0.f;
+        plain_opts.stop_min_hops = plain_opts.stop_max_hops = plain_opts.stop_confirm = 0;
+        plain_opts.stop_interval = 1;
+        plain_opts.result_cap = 0;
+        if ((rc = song::check_search_opts(ix, fallback_k, opts.ef_max, &opts.search)) != SONG_OK) return rc;
+        if ((rc = song::check_search_opts(ix, opts.k_max, opts.ef_max, &plain_opts)) != SONG_OK) return rc;
+    } else if ((rc = song::check_search_opts(ix, opts.k_max, opts.ef_max, &opts.search)) != SONG_OK) {
+        return rc;
+    }
     int metric = 0, device = 0;
     song::index_info(ix, &metric, &device);
     song_server* sv = new song_server();
     sv->ix = ix;
     sv->opts = opts;
+    sv->fallback_k = fallback_k;
+    sv->plain_opts = plain_opts;
     sv->device = device;
     sv->row_bytes = (size_t)stride * (metric == SONG_METRIC_HAMMING ? 8 : 4);
     sv->req_bytes = (int64_t)sv->row_bytes + 8 + 8 * (int64_t)opts.k_max +

@@ -13,10 +13,12 @@ void set_last_error(const std::string& msg);
 void index_info(const song_index* ix, int* metric, int* device);
 // Per-request budgets, launched on `stream` with the caller's queue counter
 // (persistent mode) and planning workspace (learned entry / cost order,
-// plan_ws_words(nq) int32); no synchronisation.
+// plan_ws_words(nq) int32); no synchronisation.  Result rows are k_stride
+// long; k_limit (<= k_stride, 0 = k_stride) bounds the requests' k, which is
+// what the options are checked against.
 int launch_mixed_on_stream(song_index* ix, const void* d_q, int nq, const int32_t* d_ks, const int32_t* d_efs,
                            int k_stride, int ef_max, const song_search_opts* opts, int32_t* d_ids, float* d_dists,
-                           int32_t* d_counter, int32_t* d_plan_ws, hipStream_t stream);
+                           int32_t* d_counter, int32_t* d_plan_ws, hipStream_t stream, int k_limit = 0);
 int64_t plan_ws_words(int64_t nq);
 // The checks a launch of these options makes, for requests up to k_max / ef_max
 // (learned searches need k <= 64, ...): SONG_OK or the launch's error.

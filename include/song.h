@@ -356,7 +356,11 @@ typedef struct song_server_opts {
     int64_t memory_budget;    /* bytes of in-flight request state, 0 = 256 MiB */
     int32_t k_max;            /* largest k accepted, 0 = 100 */
     int32_t ef_max;           /* largest queue size accepted (<= 512), 0 = 512 */
-    song_search_opts search;  /* options of every launch (mode, learned, ...) */
+    song_search_opts search;  /* options of every launch (mode, learned, ...).  A
+                                 learned search serves k <= 64; with k_max above
+                                 that, requests with k > 64 fall back to the plain
+                                 search (the same options without the learned
+                                 ones), in a second launch of the same slot. */
     int32_t resident;         /* 1 = resident kernel (see above), 0 = launch slots */
     int32_t idle_ms;          /* resident kernel: leave after this long idle, 0 = 20 */
 } song_server_opts;

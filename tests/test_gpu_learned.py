@@ -250,19 +250,30 @@ def test_result_cap_with_mixed_budgets(song, setup, mode):
             np.testing.assert_array_equal(tr[sel], r.trace)
 
 
-def test_learned_server_checks_k(song, setup):
-    """The learned search serves k <= 64: a server for larger k is refused when it is made, not at
-    its first launch."""
+@pytest.mark.parametrize("mode", [0, 1])
+def test_learned_server_falls_back_to_plain(song, setup, mode):
+    """A learned server with k_max above what the learned search serves (64): requests with k <= 64
+    get the learned search, those with k > 64 the plain one, each equal to its uniform search."""
     serving = importlib.import_module("2020_song_ex_amd.serving")
     base, q, adj, idx, model = setup
-    with pytest.raises(song.SongError, match="k <= 64"):
-        serving.Server(idx, k_max=100, learned=2, stop_threshold=2.0)
-    sv = serving.Server(idx, k_max=64, ef_max=128, learned=2, stop_threshold=2.0, stop_min_hops=8)
+    kw = dict(learned=2, stop_threshold=2.0, stop_min_hops=8, stop_interval=2)
+    rng = np.random.default_rng(7)
+    ks = rng.choice([1, 10, 100], size=q.shape[0]).astype(np.int32)
+    efs = rng.choice([64, 128], size=q.shape[0]).astype(np.int32)
+    sv = serving.Server(idx, max_batch=64, n_streams=4, k_max=100, ef_max=128, mode=mode, **kw)
     try:
-        t = sv.submit(q[0], 10, 64)
-        assert sv.wait(t, 10) is not None
+        ids, d, lat, wall = sv.run_poisson(q, ks, efs, rate_qps=100000.0, seed=4)
     finally:
         sv.close()
+    for kk in (1, 10, 100):
+        for ee in (64, 128):
+            sel = np.where((ks == kk) & (efs == ee))[0]
+            assert sel.size > 0
+            r = idx.search(q[sel], kk, ee, **(kw if kk <= 64 else {}))
+            np.testing.assert_array_equal(ids[sel, :kk], r.ids)
+            np.testing.assert_array_equal(d[sel, :kk].view(np.uint32), r.dists.view(np.uint32))
+    with pytest.raises(song.SongError, match="result_cap"):
+        serving.Server(idx, k_max=100, result_cap=16, stop_max_hops=100, **kw)  # result cap below the learned k
 
 
 def test_confirmed_stop_runs_longer(song, setup):
