@@ -313,8 +313,8 @@ def main():
         # ends by the stop network or the hop cap, not by the reference's test against the worst
         # of ef results (see include/song.h) -- which lets a short queue search long enough
         # the hop cap bounds the launch's last queries (their hops at full load set its end)
-        grid = {"ef": [64, 96], "learned": [2], "stop_threshold": [1.75, 2.0, 2.5],
-                "stop_confirm": [1, 2], "stop_max_hops": [400, 500, 600], "result_cap": [16], "stop_min_hops": [16],
+        grid = {"ef": [64, 96], "learned": [2], "stop_threshold": [1.75, 2.0, 2.25, 2.5],
+                "stop_confirm": [1, 2], "stop_max_hops": [400, 450, 500, 600], "result_cap": [16], "stop_min_hops": [16],
                 "stop_interval": [STOP_INTERVAL], "mode": [1], "waves_per_block": [4]}
         given = json.loads(args.learned_config) if args.learned_config else stored.get("learned")
         if given:

@@ -60,7 +60,7 @@ def test_learned_search(full, oracle):
     learned.fit(idx, base, extra, k=10, ef=384, stop_min_hops=16, stop_interval=8, max_evals=192,
                 entry_epochs=20, stop_epochs=4, cost_threshold=3.0)
     # the tuned configuration reaches the benchmark's recall
-    tuned = dict(learned=2, stop_threshold=2.0, stop_confirm=2, stop_max_hops=500, result_cap=16,
+    tuned = dict(learned=2, stop_threshold=2.25, stop_confirm=2, stop_max_hops=450, result_cap=16,
                  stop_min_hops=16, stop_interval=8, mode=1, waves_per_block=4)
     r = idx.search(q, 10, 64, **tuned)
     assert recall(r.ids, gt) >= 0.90
