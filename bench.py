@@ -225,7 +225,9 @@ def main():
         ef = plain_params.pop("ef")
         log(f"plain configuration from {tuned_path}: queue {ef} {plain_params}")
     if not ef and extra is not None:
-        grid = {"ef": [320, 336, 352, 368, 384, 416, 448, 512], "mode": [0], "waves_per_block": [1, 4]}
+        # the visited filter's size is a knob of both methods (the reference's default is 2^14 bits)
+        grid = {"ef": [320, 336, 352, 368, 384, 416, 448, 512], "mode": [0], "waves_per_block": [1, 4],
+                "bloom_bits": [1 << 14, 1 << 15]}
         res = autotune.tune(idx, valid, vgt, args.k, grid, tune_target, log=log)
         plain_tuned = res.table()
         if res.best is not None:
@@ -278,7 +280,8 @@ def main():
 
     pname = f"search_kernel<0,{slots_for(ef)},{lanes},{chunks},0>"
     plain_roofline = roofline_of(trace, plain_kernel_ms, pname,
-                                 f"{pname} {args.n}x{dim} nq={nq} queue={ef} k={args.k}")
+                                 f"{pname} {args.n}x{dim} nq={nq} queue={ef} k={args.k} "
+                                 f"bloom_bits={plain_params.get('bloom_bits', 1 << 14)}")
 
     # ---------------- the host-buffer path (song_search: queries H2D, ids D2H), for the record
     t0 = time.perf_counter()
@@ -313,8 +316,9 @@ def main():
         # ends by the stop network or the hop cap, not by the reference's test against the worst
         # of ef results (see include/song.h) -- which lets a short queue search long enough
         # the hop cap bounds the launch's last queries (their hops at full load set its end)
-        grid = {"ef": [64, 96], "learned": [2], "stop_threshold": [1.75, 2.0, 2.25, 2.5],
-                "stop_confirm": [1, 2], "stop_max_hops": [400, 450, 500, 600], "result_cap": [16], "stop_min_hops": [16],
+        grid = {"ef": [64], "learned": [2], "stop_threshold": [1.75, 2.0, 2.25, 2.5, 3.0],
+                "bloom_bits": [1 << 14, 1 << 15], "stop_confirm": [1, 2], "stop_max_hops": [350, 400, 450, 500],
+                "result_cap": [16], "stop_min_hops": [16],
                 "stop_interval": [STOP_INTERVAL], "mode": [1], "waves_per_block": [4]}
         given = json.loads(args.learned_config) if args.learned_config else stored.get("learned")
         if given:

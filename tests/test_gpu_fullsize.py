@@ -3,7 +3,8 @@
 on size, checked where the timed numbers come from.
 
 * A sample of queries against the oracle, bit for bit, at the tuned plain
-  queue (352): ids, distances and the reference's trace counters.
+  queue (320) with its 2^15-bit visited filter, and at the reference's
+  2^14 bits: ids, distances and the reference's trace counters.
 * The persistent kernel against the plain one on all 10,000 queries.
 * The learned search's result cap with a stop that never fires, against the
   oracle's restatement of that option.
@@ -37,18 +38,19 @@ def recall(ids, gt):
 def test_sample_matches_oracle(full, oracle):
     base, q, adj, idx, gt, _ = full
     sample = q[::50]  # 200 queries
-    r = idx.search(sample, 10, 352)
-    # 16 lanes per 128-float row; integer data sums exactly in any order
-    oi, od, ot = oracle.search(base, adj, sample, 10, 352, block=32, bloom_bits=1 << 14, group=16)
-    np.testing.assert_array_equal(r.ids, oi)
-    np.testing.assert_array_equal(r.dists.view(np.uint32), od.view(np.uint32))
-    np.testing.assert_array_equal(r.trace, ot)
+    for ef, bits in ((320, 1 << 15), (352, 1 << 14)):
+        r = idx.search(sample, 10, ef, bloom_bits=bits)
+        # 16 lanes per 128-float row; integer data sums exactly in any order
+        oi, od, ot = oracle.search(base, adj, sample, 10, ef, block=32, bloom_bits=bits, group=16)
+        np.testing.assert_array_equal(r.ids, oi)
+        np.testing.assert_array_equal(r.dists.view(np.uint32), od.view(np.uint32))
+        np.testing.assert_array_equal(r.trace, ot)
 
 
 def test_persistent_equals_plain(full):
     base, q, adj, idx, gt, _ = full
-    a = idx.search(q, 10, 352, mode=0, waves_per_block=1)
-    b = idx.search(q, 10, 352, mode=1)
+    a = idx.search(q, 10, 320, mode=0, waves_per_block=1, bloom_bits=1 << 15)
+    b = idx.search(q, 10, 320, mode=1, bloom_bits=1 << 15)
     np.testing.assert_array_equal(a.ids, b.ids)
     np.testing.assert_array_equal(a.trace, b.trace)
     assert recall(a.ids, gt) >= 0.90
@@ -60,7 +62,7 @@ def test_learned_search(full, oracle):
     learned.fit(idx, base, extra, k=10, ef=384, stop_min_hops=16, stop_interval=8, max_evals=192,
                 entry_epochs=20, stop_epochs=4, cost_threshold=3.0)
     # the tuned configuration reaches the benchmark's recall
-    tuned = dict(learned=2, stop_threshold=2.25, stop_confirm=2, stop_max_hops=450, result_cap=16,
+    tuned = dict(learned=2, stop_threshold=2.0, stop_confirm=1, stop_max_hops=450, result_cap=16, bloom_bits=1 << 15,
                  stop_min_hops=16, stop_interval=8, mode=1, waves_per_block=4)
     r = idx.search(q, 10, 64, **tuned)
     assert recall(r.ids, gt) >= 0.90
@@ -68,8 +70,8 @@ def test_learned_search(full, oracle):
     # result cap with a stop that never fires: the oracle's capped search with that result set
     sample = q[::100]
     s = idx.search(sample, 10, 64, learned=2, stop_threshold=1e30, stop_min_hops=16, stop_interval=8,
-                   stop_max_hops=150, result_cap=16)
-    oi, od, ot = oracle.search(base, adj, sample, 10, 64, block=32, bloom_bits=1 << 14, group=16, max_hops=150,
+                   stop_max_hops=150, result_cap=16, bloom_bits=1 << 15)
+    oi, od, ot = oracle.search(base, adj, sample, 10, 64, block=32, bloom_bits=1 << 15, group=16, max_hops=150,
                                result_cap=16)
     np.testing.assert_array_equal(s.ids, oi)
     np.testing.assert_array_equal(s.dists.view(np.uint32), od.view(np.uint32))
