@@ -1010,15 +1010,17 @@ __host__ __device__ constexpr size_t wave_lds_u64(int bloom_words) { return (siz
 // search with the learned stop is held to the same 80: a few per-query values
 // spill (none inside the persistent kernel's hop loop), and it runs 5 % faster
 // than at 96 VGPRs (5 waves).  (The plain search at 72 VGPRs, 7 waves, spills inside its hop
-// loop and runs slower.)  Larger rows and the feature-recording search keep
-// the compiler's own choice.
-__host__ __device__ constexpr int waves_per_simd(int qch, int learn) {
-    return qch > 2 ? 1 : learn == 0 ? 6 : learn == 1 ? 6 : 1;
+// loop and runs slower.)  With one set slot per lane (queue <= 64) the learned
+// search goes to 72 VGPRs, 7 waves: 2 % faster than 6, and 8 waves (64 VGPRs)
+// gain nothing more.  Larger rows and the feature-recording search keep the
+// compiler's own choice.
+__host__ __device__ constexpr int waves_per_simd(int qch, int learn, int s) {
+    return qch > 2 ? 1 : learn == 0 ? 6 : learn == 1 ? (s == 1 ? 7 : 6) : 1;
 }
 
 // One wave per query (the reference's one block per query, kernalGraph.h:354).
 template <int METRIC, int S, int G, int QCH, int LEARN>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(waves_per_simd(QCH, LEARN))))
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(waves_per_simd(QCH, LEARN, S))))
 search_kernel(SearchArgs P) {
     extern __shared__ unsigned long long smem[];
     unsigned long long* waves_lds = smem;
@@ -1069,7 +1071,7 @@ __device__ __forceinline__ int64_t resident_claim(const ResidentQueue& Q) {
 // resident queue the waves serve the request ring instead (resident_claim),
 // publishing each request's results before its done flag.
 template <int METRIC, int S, int G, int QCH, int LEARN>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(waves_per_simd(QCH, LEARN))))
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(waves_per_simd(QCH, LEARN, S))))
 persistent_kernel(SearchArgs P) {
     extern __shared__ unsigned long long smem[];
     unsigned long long* waves_lds = smem;
