@@ -83,3 +83,23 @@ def test_autotuner_pick_is_deterministic():
     assert at.pick(tried, 0.96) is tried[3]
     assert at.pick(tried, 0.99) is None
     assert at.pick(list(reversed(tried)), 0.90) is tried[2]  # order decides ties
+
+
+def test_autotuner_finalists():
+    """The finalists are the fastest candidates that meet the recall target, fastest first;
+    the final pick is the lowest median of their interleaved launch times."""
+    import importlib
+    at = importlib.import_module("2020_song_ex_amd.autotune")
+    C = at.Candidate
+    tried = [C({"ef": 1}, recall=0.85, qps=9e6), C({"ef": 2}, recall=0.91, qps=5e6),
+             C({"ef": 3}, recall=0.95, qps=6e6), C({"ef": 4}, recall=0.97, qps=5e6),
+             C({"ef": 5}, recall=0.99, qps=1e6)]
+    f = at.finalists(tried, 0.90, 3)
+    assert f == [tried[2], tried[1], tried[3]]  # 6e6, then the 5e6 tie in grid order
+    assert at.finalists(tried, 0.999, 3) == []
+    assert at.finalists(tried, 0.90, 10) == [tried[2], tried[1], tried[3], tried[4]]
+    # one slow outlier does not decide: medians 2.0, 1.9, 2.1
+    times = [[2.0, 2.0, 1.5, 2.2, 2.0], [1.9, 3.0, 1.9, 1.8, 1.9], [2.1, 2.1, 2.1, 2.1, 2.1]]
+    assert at.pick_final(f, times) is tried[1]
+    assert at.pick_final(f, [[1.0], [1.0], [0.5]]) is tried[3]
+    assert at.pick_final(f, [[1.0], [1.0], [1.0]]) is tried[2]  # equal medians keep the earlier
