@@ -346,6 +346,9 @@ def main():
                            "ms_per_step": round(lms_step, 4), "kernel_ms": round(lkernel_ms, 4),
                            "kernel_ms_std": round(lkernel_std, 4), "qps_kernel": round(nq / (lkernel_ms / 1e3), 1),
                            "expanded_mean": float(ltr[:, 2].mean()),
+                           # the launch ends with its longest searches
+                           "expanded_p50_p95_max": [float(np.percentile(ltr[:, 2], 50)),
+                                                    float(np.percentile(ltr[:, 2], 95)), int(ltr[:, 2].max())],
                            "learned_stops": float((ltr[:, 11] == 2).mean()),
                            "hop_capped": float((ltr[:, 11] == 3).mean()),
                            "speedup_vs_plain": round(lqps / plain_qps, 3),
