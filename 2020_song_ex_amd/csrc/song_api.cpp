@@ -46,6 +46,20 @@ double ms_since(std::chrono::steady_clock::time_point a) {
 
 bool is_pow2(int64_t v) { return v > 0 && (v & (v - 1)) == 0; }
 
+// True when the first `cols` floats of each of `rows` rows (row pitch
+// `pitch` floats) are all finite.  NaN distances would order as keys above
+// every finite distance, where the reference's float scans treat them by
+// position (a NaN never compares smaller, gpuHeap.h:28,41), so non-finite
+// vectors and queries are refused instead of searched.
+bool rows_finite(const float* p, int64_t rows, int64_t cols, int64_t pitch) {
+    for (int64_t r = 0; r < rows; r++) {
+        const float* row = p + r * pitch;
+        for (int64_t c = 0; c < cols; c++)
+            if (!std::isfinite(row[c])) return false;
+    }
+    return true;
+}
+
 int ilog2(int64_t v) {
     int r = 0;
     while ((int64_t(1) << r) < v) r++;
@@ -472,6 +486,8 @@ int song_index_create(const void* vecs, const int32_t* adj, int64_t n, int32_t d
     for (int64_t i = 0; i < n * block; i++) {
         if (adj[i] < -1 || adj[i] >= n) return fail(SONG_ERR_ARG, "adjacency entry out of range at " + std::to_string(i));
     }
+    if (metric != SONG_METRIC_HAMMING && !rows_finite((const float*)vecs, n, dim, dim))
+        return fail(SONG_ERR_ARG, "vectors must be finite (NaN or Inf found)");
     int nd = 0;
     HIP_TRY(hipGetDeviceCount(&nd), "hipGetDeviceCount");
     if (device < 0 || device >= nd) return fail(SONG_ERR_ARG, "device out of range");
@@ -556,6 +572,8 @@ int song_search(song_index* ix, const void* queries, int32_t nq, int32_t k, int3
     Plan p;
     int rc = make_plan(ix, nq, k, ef, opts, &p);
     if (rc) return rc;
+    if (ix->metric != SONG_METRIC_HAMMING && !rows_finite((const float*)queries, nq, ix->dim, ix->dim))
+        return fail(SONG_ERR_ARG, "queries must be finite (NaN or Inf found)");
     std::lock_guard<std::mutex> lock(ix->mu);
     HIP_TRY(hipSetDevice(ix->device), "hipSetDevice");
     song_timing tm;
@@ -615,22 +633,21 @@ int song_search(song_index* ix, const void* queries, int32_t nq, int32_t k, int3
     return SONG_OK;
 }
 
-int song_search_device(song_index* ix, const void* d_queries, int32_t nq, int32_t k, int32_t ef,
-                       const song_search_opts* opts, int32_t* d_ids, float* d_dists, song_query_trace* d_trace,
-                       song_timing* timing_out) {
-    if (!ix) return fail(SONG_ERR_ARG, "null index");
-    if (nq > 0 && (!d_queries || !d_ids)) return fail(SONG_ERR_ARG, "null buffer");
-    Plan p;
-    int rc = make_plan(ix, nq, k, ef, opts, &p);
-    if (rc) return rc;
-    std::lock_guard<std::mutex> lock(ix->mu);
-    HIP_TRY(hipSetDevice(ix->device), "hipSetDevice");
+}  // extern "C"
+
+namespace {
+
+// The batch loop of song_search_device over the
+// caller's device buffers: `counter` is the persistent kernel's queue head and
+// `plan_ws` the planning pass's workspace (large enough for one batch).
+// Called with ix->mu held and the index's device current.
+int search_device_batches(song_index* ix, const Plan& p, const void* d_queries, int32_t nq, int32_t k,
+                          int32_t* d_ids, float* d_dists, song_query_trace* d_trace, int32_t* counter,
+                          int32_t* plan_ws, song_timing* timing_out) {
     song_timing tm;
     std::memset(&tm, 0, sizeof tm);
     tm.ef_used = p.ef;
     const size_t rb = row_bytes(ix);
-    if (needs_plan(p) && (rc = ensure_plan_ws(ix, std::min<int64_t>(p.batch, std::max<int32_t>(nq, 1)), p.n_entry)))
-        return rc;
     auto t0 = std::chrono::steady_clock::now();
     HIP_TRY(hipEventRecord(ix->ev[1], ix->stream), "hipEventRecord");
     int bi = 0;
@@ -639,7 +656,8 @@ int song_search_device(song_index* ix, const void* d_queries, int32_t nq, int32_
         song::SearchArgs a = make_args(ix, p, (const char*)d_queries + off * rb, bs, k, d_ids + off * k,
                                        d_dists ? d_dists + off * k : nullptr,
                                        d_trace ? (int32_t*)(d_trace + off) : nullptr);
-        HIP_TRY(run_plan(ix, p, a, ix->d_plan, ix->stream), "planning pass");
+        a.queue_head = p.persistent ? counter : nullptr;
+        HIP_TRY(run_plan(ix, p, a, plan_ws, ix->stream), "planning pass");
         HIP_TRY(song::launch_search(a, ix->metric, p.S, p.G, p.qch, p.shape, ix->stream), "search kernel launch");
     }
     HIP_TRY(hipEventRecord(ix->ev[2], ix->stream), "hipEventRecord");
@@ -653,6 +671,26 @@ int song_search_device(song_index* ix, const void* d_queries, int32_t nq, int32_
     return SONG_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+int song_search_device(song_index* ix, const void* d_queries, int32_t nq, int32_t k, int32_t ef,
+                       const song_search_opts* opts, int32_t* d_ids, float* d_dists, song_query_trace* d_trace,
+                       song_timing* timing_out) {
+    if (!ix) return fail(SONG_ERR_ARG, "null index");
+    if (nq > 0 && (!d_queries || !d_ids)) return fail(SONG_ERR_ARG, "null buffer");
+    Plan p;
+    int rc = make_plan(ix, nq, k, ef, opts, &p);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lock(ix->mu);
+    HIP_TRY(hipSetDevice(ix->device), "hipSetDevice");
+    if (needs_plan(p) && (rc = ensure_plan_ws(ix, std::min<int64_t>(p.batch, std::max<int32_t>(nq, 1)), p.n_entry)))
+        return rc;
+    return search_device_batches(ix, p, d_queries, nq, k, d_ids, d_dists, d_trace, ix->d_counter, ix->d_plan,
+                                 timing_out);
+}
+
 int song_search_device_mixed(song_index* ix, const void* d_queries, int32_t nq, const int32_t* d_ks,
                              const int32_t* d_efs, int32_t k_max, int32_t ef_max, const song_search_opts* opts,
                              int32_t* d_ids, float* d_dists, song_query_trace* d_trace, song_timing* timing_out) {
@@ -663,6 +701,20 @@ int song_search_device_mixed(song_index* ix, const void* d_queries, int32_t nq, 
     if (rc) return rc;
     std::lock_guard<std::mutex> lock(ix->mu);
     HIP_TRY(hipSetDevice(ix->device), "hipSetDevice");
+    // every budget is checked on the host before the launch: 1 <= k <= k_max
+    // and max(ef, k) <= ef_max (the kernel sizes its sets and rows by them)
+    if (nq > 0) {
+        std::vector<int32_t> hk((size_t)nq), he((size_t)nq);
+        HIP_TRY(hipMemcpyAsync(hk.data(), d_ks, (size_t)nq * 4, hipMemcpyDeviceToHost, ix->stream), "budgets D2H");
+        HIP_TRY(hipMemcpyAsync(he.data(), d_efs, (size_t)nq * 4, hipMemcpyDeviceToHost, ix->stream), "budgets D2H");
+        HIP_TRY(hipStreamSynchronize(ix->stream), "budgets D2H");
+        for (int32_t i = 0; i < nq; i++) {
+            if (hk[i] < 1 || hk[i] > k_max)
+                return fail(SONG_ERR_ARG, "query " + std::to_string(i) + ": k must be in [1, k_max]");
+            if (std::max(he[i], hk[i]) > p.ef)
+                return fail(SONG_ERR_ARG, "query " + std::to_string(i) + ": max(ef, k) must be <= ef_max");
+        }
+    }
     song_timing tm;
     std::memset(&tm, 0, sizeof tm);
     tm.ef_used = p.ef;
@@ -923,6 +975,8 @@ int song_index_bruteforce(song_index* ix, const void* queries, int32_t nq, int32
     if (k < 1 || k > 128) return fail(SONG_ERR_ARG, "k must be in [1, 128]");
     if (nq < 0 || (nq > 0 && (!queries || !ids_out))) return fail(SONG_ERR_ARG, "bad query buffers");
     if (nq == 0) return SONG_OK;
+    if (ix->metric != SONG_METRIC_HAMMING && !rows_finite((const float*)queries, nq, ix->dim, ix->dim))
+        return fail(SONG_ERR_ARG, "queries must be finite (NaN or Inf found)");
     std::lock_guard<std::mutex> lock(ix->mu);
     HIP_TRY(hipSetDevice(ix->device), "hipSetDevice");
     const size_t rb = row_bytes(ix);
@@ -1052,6 +1106,8 @@ int song_memcpy_d2h(void* dst, const void* src, int64_t bytes) {
 
 int song_queries_to_device(song_index* ix, const void* queries, int32_t nq, void** d_out) {
     if (!ix || !d_out || (nq > 0 && !queries) || nq < 0) return fail(SONG_ERR_ARG, "bad arguments");
+    if (ix->metric != SONG_METRIC_HAMMING && !rows_finite((const float*)queries, nq, ix->dim, ix->dim))
+        return fail(SONG_ERR_ARG, "queries must be finite (NaN or Inf found)");
     std::lock_guard<std::mutex> lock(ix->mu);
     HIP_TRY(hipSetDevice(ix->device), "hipSetDevice");
     const size_t rb = row_bytes(ix);

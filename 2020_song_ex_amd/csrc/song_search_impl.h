@@ -687,10 +687,12 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
     const uint32_t wmask = (uint32_t)P.bloom_words - 1u;
     const int wshift = P.bloom_shift;
     const uint32_t nbits_mask = (uint32_t)P.bloom_words * 64u - 1u;
-    // per-query budgets (serving / persistent queues mix them); ef >= k
-    // (clamped so a bad device-side budget cannot overrun the register sets)
-    const int k = min(P.ks ? P.ks[qid] : P.k, P.k_stride);
-    const int ef = min(P.efs ? max(P.efs[qid], k) : P.ef, 64 * S);
+    // per-query budgets (serving / persistent queues mix them); ef >= k.  The
+    // host checks every budget before the launch (1 <= k <= k_stride,
+    // max(ef, k) <= P.ef); the clamps only keep a budget that bypassed those
+    // checks inside the output row and the register sets.
+    const int k = max(1, min(P.ks ? P.ks[qid] : P.k, P.k_stride));
+    const int ef = min(P.efs ? max(P.efs[qid], k) : P.ef, min(P.ef, 64 * S));
 
     for (int w = lane; w < P.bloom_words; w += 64) bloom[w] = 0ull;
 

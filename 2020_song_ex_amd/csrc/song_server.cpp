@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -75,6 +76,7 @@ struct song_server {
     song_index* ix = nullptr;
     song_server_opts opts{};
     int device = 0;
+    bool float_rows = true;  // query rows are floats (L2, IP), not Hamming codes
     size_t row_bytes = 0;
     int64_t req_bytes = 0;
     int capacity = 0;  // requests per slot
@@ -474,6 +476,7 @@ int song_server_create(song_index* ix, const song_server_opts* o, song_server** 
     sv->fallback_k = fallback_k;
     sv->plain_opts = plain_opts;
     sv->device = device;
+    sv->float_rows = metric != SONG_METRIC_HAMMING;
     sv->row_bytes = (size_t)stride * (metric == SONG_METRIC_HAMMING ? 8 : 4);
     sv->req_bytes = (int64_t)sv->row_bytes + 8 + 8 * (int64_t)opts.k_max +
                     (opts.search.learned ? 4 * song::plan_ws_words(1) : 0);
@@ -535,6 +538,11 @@ static int submit_at(song_server* sv, const void* query, int32_t k, int32_t ef, 
     if (!sv || !query || !ticket) return sfail(SONG_ERR_ARG, "null argument");
     if (k < 1 || k > sv->opts.k_max) return sfail(SONG_ERR_ARG, "k out of range for this server");
     if (std::max(ef, k) > sv->opts.ef_max) return sfail(SONG_ERR_ARG, "queue size above the server's ef_max");
+    if (sv->float_rows) {
+        const float* qf = (const float*)query;
+        for (size_t i = 0; i < sv->row_bytes / 4; i++)
+            if (!std::isfinite(qf[i])) return sfail(SONG_ERR_ARG, "query must be finite (NaN or Inf found)");
+    }
     song::Request* r = new song::Request();
     r->ticket = sv->next_ticket.fetch_add(1);
     r->k = k;

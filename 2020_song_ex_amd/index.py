@@ -56,6 +56,32 @@ def nsw_build(vecs, metric="l2", block=REF_BLOCK, fixed_deg=None, search_deg=Non
     return adj
 
 
+def cpu_search(vecs, adj, queries, k, ef, *, n_threads=1, n_seeds=0, best_first=True):
+    """The reference's CPU search, Graph::search (graph.h:33-115), through song_cpu_search on
+    the host.  Returns (ids [nq, k], dists, trace [nq, 12], per-query ms [nq], search ms); ids
+    best first unless best_first=False (Graph::search's own worst-first order)."""
+    v = np.ascontiguousarray(vecs, dtype=np.float32)
+    q = np.ascontiguousarray(queries, dtype=np.float32).reshape(-1, v.shape[1])
+    a = np.ascontiguousarray(adj, dtype=np.int32)
+    nq = q.shape[0]
+    ids = np.empty((nq, k), np.int32)
+    d = np.empty((nq, k), np.float32)
+    tr = np.empty((nq, 12), np.int32)
+    per = np.empty(nq, np.float64)
+    total = ctypes.c_double()
+    N.check(N.lib().song_cpu_search(N.ptr(v), N.ptr(a), v.shape[0], v.shape[1], a.shape[1], N.ptr(q), nq, k,
+                                    ef, n_seeds, n_threads, N.ptr(ids), N.ptr(d), N.ptr(tr), N.ptr(per),
+                                    ctypes.byref(total)), "song_cpu_search")
+    if best_first:
+        # worst first with -1 padding at the end -> best first, padding kept at the end
+        cnt = tr[:, 10]
+        for i in range(nq):
+            c = int(cnt[i])
+            ids[i, :c] = ids[i, :c][::-1].copy()
+            d[i, :c] = d[i, :c][::-1].copy()
+    return ids, d, tr, per, total.value
+
+
 def graph_save(path, adj, metric="l2", dim=0):
     adj = np.ascontiguousarray(adj, dtype=np.int32)
     N.check(N.lib().song_graph_save(os.fsencode(path), N.ptr(adj), adj.shape[0], adj.shape[1],

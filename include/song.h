@@ -244,6 +244,23 @@ int song_search_device(song_index* index, const void* d_queries, int32_t nq, int
                        const song_search_opts* opts, int32_t* d_ids, float* d_dists,
                        song_query_trace* d_trace, song_timing* timing_out);
 
+/* The reference's CPU search baseline on the host: Graph::search
+ * (graph.h:33-115) with the same ids and QueryTrace counters, as bench_cpu
+ * runs it (bench_cpu.cpp:444-452), with the queries optionally split over
+ * n_threads host threads.  Squared L2 only (summed in index order,
+ * distance.h:5-12); the graph's rows have `block` slots (BLOCK_SIZE,
+ * config.h:9) and n_seeds evenly spaced entry points (0 = 4, N_MULTIPROBE).
+ * This is the CPU method of the bench_cpu driver; the GPU entry points never
+ * fall back to it.
+ *   ids_out: [nq x k] in Graph::search's own order, WORST first, -1 padded
+ *   dists_out, trace_out, per_query_ms_out (each query's wall time),
+ *   search_ms_out (wall time of the whole search, argument checks excluded):
+ *   nullable. */
+int song_cpu_search(const float* vecs, const int32_t* adj, int64_t n, int32_t dim, int32_t block,
+                    const float* queries, int32_t nq, int32_t k, int32_t ef, int32_t n_seeds, int32_t n_threads,
+                    int32_t* ids_out, float* dists_out, song_query_trace* trace_out, double* per_query_ms_out,
+                    double* search_ms_out);
+
 /* Exact k nearest neighbours by brute force on the device, over the index's
  * resident vectors (ground truth for recall).  Replaces the reference's CPU
  * brute force (bench_cpu.cpp:158-175) and numpy compute_gt
@@ -254,8 +271,10 @@ int song_index_bruteforce(song_index* index, const void* queries, int32_t nq, in
 
 /* Device-resident search with a budget per query (mixed k and queue sizes in
  * one launch, as a serving queue holds them).  d_ks / d_efs: int32 [nq] on the
- * device; each ef is raised to its k; k_max >= every k, ef_max >= every ef
- * (<= 512).  Output rows are k_max long, -1 padded.  No counterpart in the
+ * device; each ef is raised to its k.  Required, and checked on the host
+ * (the budgets are copied back) before anything is launched:
+ * 1 <= k <= k_max and max(ef, k) <= max(ef_max, k_max) <= 512; otherwise
+ * SONG_ERR_ARG.  Output rows are k_max long, -1 padded.  No counterpart in the
  * reference (one k and ef per gpu_search call, kernalGraph.h:307-309). */
 int song_search_device_mixed(song_index* index, const void* d_queries, int32_t nq, const int32_t* d_ks,
                              const int32_t* d_efs, int32_t k_max, int32_t ef_max,

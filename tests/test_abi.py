@@ -103,3 +103,13 @@ def test_autotuner_finalists():
     assert at.pick_final(f, times) is tried[1]
     assert at.pick_final(f, [[1.0], [1.0], [0.5]]) is tried[3]
     assert at.pick_final(f, [[1.0], [1.0], [1.0]]) is tried[2]  # equal medians keep the earlier
+
+
+@pytest.mark.parametrize("bad", [np.nan, np.inf, -np.inf])
+def test_index_rejects_non_finite_vectors(song, bad):
+    """Checked on the host before any device call: a NaN distance would order above every
+    finite key, where the reference's float scans treat it by position."""
+    data = np.zeros((10, 4), np.float32)
+    data[7, 2] = bad
+    with pytest.raises(song.SongError, match="finite"):
+        song.Index(data, np.full((10, 8), -1, np.int32))
