@@ -52,7 +52,7 @@ EXPORTS = (
     "song_search_device_mixed", "song_dev_copy", "song_index_set_model",
     "song_learn_trace", "song_mlp_train", "song_mlp_forward", "song_server_create", "song_server_submit",
     "song_server_poll", "song_server_wait", "song_server_stats_get", "song_server_destroy",
-    "song_server_run_poisson",
+    "song_server_run_poisson", "song_search_device_static",
     "song_cpu_search",
 )
 
@@ -91,6 +91,7 @@ def lib():
     L.song_search_device.argtypes = [_P, _P, _I32, _I32, _I32, _P, _P, _P, _P, _P]
     L.song_cpu_search.argtypes = [_P, _P, _I64, _I32, _I32, _P, _I32, _I32, _I32, _I32, _I32, _P, _P, _P, _P,
                                   _P]
+    L.song_search_device_static.argtypes = [_P, _P, _I32, _I32, _I32, _P, _P, _P, _P, _P]
     L.song_dev_alloc.argtypes = [_I32, _I64, ctypes.POINTER(_P)]
     L.song_dev_free.argtypes = [_P]
     L.song_memcpy_h2d.argtypes = [_P, _P, _I64]

@@ -637,7 +637,7 @@ int song_search(song_index* ix, const void* queries, int32_t nq, int32_t k, int3
 
 namespace {
 
-// The batch loop of song_search_device over the
+// The batch loop of song_search_device and song_search_device_static over the
 // caller's device buffers: `counter` is the persistent kernel's queue head and
 // `plan_ws` the planning pass's workspace (large enough for one batch).
 // Called with ix->mu held and the index's device current.
@@ -671,6 +671,36 @@ int search_device_batches(song_index* ix, const Plan& p, const void* d_queries, 
     return SONG_OK;
 }
 
+// Scratch of song_search_device_static, one copy per GPU for the life of the
+// process: words [0, 64) hold the persistent kernel's queue head (zeroed by a
+// kernel on the stream before each launch), the rest the planning pass's
+// workspace, entries | cost | order, for one batch (every word it reads is
+// written first in the same call).  In host code `scratch` only names the
+// symbol; its device address comes from hipGetSymbolAddress, per device.
+constexpr int64_t kScratchHeadWords = 64;
+constexpr int64_t kScratchPlanWords = (int64_t)SONG_STATIC_MAX_PLAN_QUERIES * (64 + 2);
+static __device__ __align__(256) int32_t scratch[kScratchHeadWords + kScratchPlanWords];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+int32_t* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// The device address of `scratch` on `device`, resolved once per device with
+// that device current (each GPU holds its own copy at its own address).
+int scratch_on_device(int device, int32_t** out) {
+    if (device < 0 || device >= kMaxScratchDevices) return fail(SONG_ERR_ARG, "device id out of range for scratch");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        HIP_TRY(hipSetDevice(device), "hipSetDevice");
+        void* ptr = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (!ptr) return fail(SONG_ERR_HIP, "hipGetSymbolAddress(scratch) returned null");
+        g_scratch_ptr[device] = (int32_t*)ptr;
+    }
+    *out = g_scratch_ptr[device];
+    return SONG_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -688,6 +718,27 @@ int song_search_device(song_index* ix, const void* d_queries, int32_t nq, int32_
     if (needs_plan(p) && (rc = ensure_plan_ws(ix, std::min<int64_t>(p.batch, std::max<int32_t>(nq, 1)), p.n_entry)))
         return rc;
     return search_device_batches(ix, p, d_queries, nq, k, d_ids, d_dists, d_trace, ix->d_counter, ix->d_plan,
+                                 timing_out);
+}
+
+int song_search_device_static(song_index* ix, const void* d_queries, int32_t nq, int32_t k, int32_t ef,
+                              const song_search_opts* opts, int32_t* d_ids, float* d_dists,
+                              song_query_trace* d_trace, song_timing* timing_out) {
+    if (!ix) return fail(SONG_ERR_ARG, "null index");
+    if (nq > 0 && (!d_queries || !d_ids)) return fail(SONG_ERR_ARG, "null buffer");
+    Plan p;
+    int rc = make_plan(ix, nq, k, ef, opts, &p);
+    if (rc) return rc;
+    const int64_t batch = std::min<int64_t>(p.batch, std::max<int32_t>(nq, 1));
+    if (needs_plan(p) && plan_words(batch, p.n_entry) > kScratchPlanWords)
+        return fail(SONG_ERR_ARG, "song_search_device_static: with the entry network or the cost order a batch "
+                                  "holds at most " + std::to_string(kScratchPlanWords / (p.n_entry + 2)) +
+                                  " queries for this n_seeds (set opts->batch)");
+    std::lock_guard<std::mutex> lock(ix->mu);
+    int32_t* s = nullptr;
+    if ((rc = scratch_on_device(ix->device, &s))) return rc;
+    HIP_TRY(hipSetDevice(ix->device), "hipSetDevice");
+    return search_device_batches(ix, p, d_queries, nq, k, d_ids, d_dists, d_trace, s, s + kScratchHeadWords,
                                  timing_out);
 }
 

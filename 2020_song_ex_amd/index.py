@@ -225,16 +225,18 @@ class Index:
                       d_dists: N.DeviceBuffer | None = None, d_trace: N.DeviceBuffer | None = None, *,
                       bloom_bits=0, batch=0, n_seeds=0, waves_per_block=0, lanes_per_row=0, mode=0,
                       waves_per_cu=0, learned=0, stop_threshold=0.0, stop_min_hops=0, stop_interval=1,
-                      stop_max_hops=0, stop_confirm=0, result_cap=0) -> dict:
-        """Search with every buffer resident on the device; returns the timing dict."""
+                      stop_max_hops=0, stop_confirm=0, result_cap=0, no_alloc=False) -> dict:
+        """Search with every buffer resident on the device; returns the timing dict.
+        no_alloc: through song_search_device_static (no device allocation; its scratch is a
+        static array per GPU, so calls on one GPU must not overlap)."""
         o = N.make_opts(bloom_bits, batch, n_seeds, waves_per_block, lanes_per_row, mode, waves_per_cu,
                         learned, stop_threshold, stop_min_hops, stop_interval, stop_max_hops, stop_confirm,
                         result_cap)
         tm = N.SongTiming()
-        N.check(N.lib().song_search_device(self._h, dq.ptr, nq, k, ef, ctypes.byref(o), d_ids.ptr,
-                                           d_dists.ptr if d_dists else None,
-                                           d_trace.ptr if d_trace else None, ctypes.byref(tm)),
-                "song_search_device")
+        fn = N.lib().song_search_device_static if no_alloc else N.lib().song_search_device
+        N.check(fn(self._h, dq.ptr, nq, k, ef, ctypes.byref(o), d_ids.ptr,
+                   d_dists.ptr if d_dists else None, d_trace.ptr if d_trace else None, ctypes.byref(tm)),
+                "song_search_device_static" if no_alloc else "song_search_device")
         return {f: getattr(tm, f) for f, _ in N.SongTiming._fields_}
 
     def search_device_mixed(self, dq: N.DeviceBuffer, nq, d_ks: N.DeviceBuffer, d_efs: N.DeviceBuffer,

@@ -244,6 +244,23 @@ int song_search_device(song_index* index, const void* d_queries, int32_t nq, int
                        const song_search_opts* opts, int32_t* d_ids, float* d_dists,
                        song_query_trace* d_trace, song_timing* timing_out);
 
+/* song_search_device without any device allocation: the same arguments,
+ * kernels and results.  No counterpart in the reference; it is for callers
+ * that cannot call hipMalloc (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid allocation).  Its scratch -- the persistent
+ * kernel's queue head and the planning pass's workspace -- is one static
+ * device array per GPU of 256 + 264 * SONG_STATIC_MAX_PLAN_QUERIES bytes
+ * (about 4.3 MB), held until the process ends.  Limit: with learned bit 0 or
+ * bit 2 a batch (opts->batch, or nq when 0) holds at most
+ * 66 * SONG_STATIC_MAX_PLAN_QUERIES / (n_entry + 2) queries (n_entry = n_seeds,
+ * twice that with learned bit 3); a larger one returns SONG_ERR_ARG.  Every
+ * call on one GPU shares that GPU's array: calls that overlap on one GPU (any
+ * index, thread or stream) race, so serialise them. */
+#define SONG_STATIC_MAX_PLAN_QUERIES 16384
+int song_search_device_static(song_index* index, const void* d_queries, int32_t nq, int32_t k, int32_t ef,
+                              const song_search_opts* opts, int32_t* d_ids, float* d_dists,
+                              song_query_trace* d_trace, song_timing* timing_out);
+
 /* The reference's CPU search baseline on the host: Graph::search
  * (graph.h:33-115) with the same ids and QueryTrace counters, as bench_cpu
  * runs it (bench_cpu.cpp:444-452), with the queries optionally split over
