@@ -26,10 +26,12 @@
 #include <string>
 #include <vector>
 
+#include "cli_common.hpp"
 #include "song_graph.hpp"
 
 namespace fs = std::filesystem;
 using clk = std::chrono::steady_clock;
+using namespace song::cli;
 
 namespace {
 
@@ -59,108 +61,6 @@ Options parse(int argc, char** argv) {
     return o;
 }
 
-std::vector<int> int_list(const std::string& text) {
-    std::vector<int> v;
-    std::stringstream ss(text);
-    std::string tok;
-    while (std::getline(ss, tok, ','))
-        if (!tok.empty()) v.push_back(std::stoi(tok));
-    if (v.empty()) v.push_back(song::pq_size);
-    std::sort(v.begin(), v.end());
-    v.erase(std::unique(v.begin(), v.end()), v.end());
-    return v;
-}
-
-// .fvecs rows (parser.h:10-22), first `limit` rows.
-std::vector<float> read_fvecs(const std::string& path, int limit, int& rows, int& dim) {
-    std::ifstream f(path, std::ios::binary);
-    if (!f) throw song::Error("cannot open " + path);
-    std::vector<float> out;
-    rows = 0;
-    dim = 0;
-    int32_t d;
-    while (rows < limit && f.read((char*)&d, 4)) {
-        if (dim == 0) dim = d;
-        if (d != dim) throw song::Error("inconsistent dim in " + path);
-        size_t off = out.size();
-        out.resize(off + d);
-        if (!f.read((char*)(out.data() + off), (std::streamsize)d * 4)) throw song::Error("short row in " + path);
-        rows++;
-    }
-    return out;
-}
-
-// int32 .npy [rows x cols] ground truth (bench_gpu.cu:163-219 reads the same layout).
-std::vector<std::vector<int>> load_gt(const std::string& path, int nq, int k) {
-    std::ifstream in(path, std::ios::binary);
-    if (!in) throw song::Error("failed to open gt npy: " + path);
-    char magic[6];
-    in.read(magic, 6);
-    if (!in || std::string(magic, 6) != std::string("\x93NUMPY", 6)) throw song::Error("invalid npy magic: " + path);
-    unsigned char ver[2];
-    in.read((char*)ver, 2);
-    uint32_t hlen = 0;
-    if (ver[0] == 1) {
-        uint16_t h;
-        in.read((char*)&h, 2);
-        hlen = h;
-    } else {
-        in.read((char*)&hlen, 4);
-    }
-    std::string header(hlen, '\0');
-    in.read(header.data(), hlen);
-    size_t lp = header.find('(', header.find("shape")), rp = header.find(')', lp);
-    if (lp == std::string::npos || rp == std::string::npos) throw song::Error("failed to parse npy shape: " + path);
-    std::string shape = header.substr(lp + 1, rp - lp - 1);
-    for (char& c : shape)
-        if (c < '0' || c > '9') c = ' ';
-    std::istringstream iss(shape);
-    int rows = 0, cols = 0;
-    iss >> rows >> cols;
-    if (rows < nq || cols < k) throw song::Error("gt npy smaller than requested nq/k: " + path);
-    std::vector<int32_t> raw((size_t)rows * cols);
-    in.read((char*)raw.data(), (std::streamsize)raw.size() * 4);
-    if (!in) throw song::Error("failed to read npy payload: " + path);
-    std::vector<std::vector<int>> gt(nq, std::vector<int>(k));
-    for (int r = 0; r < nq; r++)
-        for (int c = 0; c < k; c++) gt[r][c] = raw[(size_t)r * cols + c];
-    return gt;
-}
-
-struct Stats {
-    double min = 0, max = 0, mean = 0, stddev = 0, p50 = 0, p90 = 0, p95 = 0, p99 = 0;
-};
-
-// Linear interpolation at q*(N-1); population std (bench_gpu.cu:127-156).
-Stats summarize(std::vector<double> v) {
-    Stats s;
-    if (v.empty()) return s;
-    std::sort(v.begin(), v.end());
-    auto q = [&](double p) {
-        if (v.size() == 1) return v[0];
-        double pos = p * (v.size() - 1);
-        size_t lo = (size_t)std::floor(pos), hi = (size_t)std::ceil(pos);
-        return v[lo] * (1.0 - (pos - lo)) + v[hi] * (pos - lo);
-    };
-    s.min = v.front();
-    s.max = v.back();
-    s.mean = std::accumulate(v.begin(), v.end(), 0.0) / v.size();
-    double var = 0;
-    for (double x : v) var += (x - s.mean) * (x - s.mean);
-    s.stddev = std::sqrt(var / v.size());
-    s.p50 = q(0.5);
-    s.p90 = q(0.9);
-    s.p95 = q(0.95);
-    s.p99 = q(0.99);
-    return s;
-}
-
-std::string pq_tag(int pq) {
-    std::ostringstream o;
-    o << "pq" << std::setw(4) << std::setfill('0') << pq;
-    return o.str();
-}
-
 struct Row {
     std::string method;
     int pq = 0;
@@ -184,16 +84,7 @@ Row run_variant(const std::string& method, const Options& o, int pq, Fn&& search
     }
     std::vector<double> rec(o.nq, 0.0);
     if (!gt.empty())
-        for (int q = 0; q < o.nq; q++) {
-            int hit = 0;
-            for (int j = 0; j < o.k; j++)
-                for (int r = 0; r < o.k; r++)
-                    if (gt[q][r] == ids[(size_t)q * o.k + j]) {
-                        hit++;
-                        break;
-                    }
-            rec[q] = (double)hit / o.k;
-        }
+        for (int q = 0; q < o.nq; q++) rec[q] = recall_of(ids.data() + (size_t)q * o.k, gt[q], o.k);
     fs::path dir(o.out_dir);
     Row row;
     row.method = method;
@@ -201,13 +92,7 @@ Row run_variant(const std::string& method, const Options& o, int pq, Fn&& search
     row.ids_path = (dir / (method + "_" + pq_tag(pq) + ".ids")).string();
     row.qstats_path = (dir / (method + "_" + pq_tag(pq) + "_query_stats.csv")).string();
     row.bstats_path = (dir / (method + "_" + pq_tag(pq) + "_batch_stats.csv")).string();
-    {
-        std::ofstream f(row.ids_path);
-        for (int q = 0; q < o.nq; q++) {
-            for (int j = 0; j < o.k; j++) f << (j ? " " : "") << ids[(size_t)q * o.k + j];
-            f << '\n';
-        }
-    }
+    write_ids(row.ids_path, ids, o.nq, o.k);
     {
         std::ofstream f(row.bstats_path);
         f << "pq,batch_index,batch_offset,batch_size,h2d_ms,memset_ms,kernel_ms,d2h_ms,total_ms,per_query_total_ms\n";
@@ -293,7 +178,7 @@ void write_summary(const fs::path& path, const std::vector<Row>& rows) {
 int main(int argc, char** argv) {
     try {
         Options o = parse(argc, argv);
-        const std::vector<int> pqs = int_list(o.pq_list);
+        const std::vector<int> pqs = int_list(o.pq_list, song::pq_size);
         auto t0 = clk::now();
         int nb = 0, dim = 0, nqr = 0, qdim = 0;
         std::vector<float> base = read_fvecs(o.base, o.n, nb, dim);

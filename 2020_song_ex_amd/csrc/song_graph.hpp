@@ -176,6 +176,41 @@ inline KernelGraph build_kernel_graph(std::vector<float> data, int n, int dim, c
     return g;
 }
 
+// Graph (graph.h:11-115): the CPU graph and its search, the reference's CPU
+// baseline.  search() returns the result ids worst first, as
+// Graph::search's vector holds them.
+struct Graph {
+    int num_nodes = 0;
+    int dim = 0;
+    int block = BLOCK_SIZE;
+    std::vector<float> data;
+    std::vector<int> adj;
+
+    Graph(int n, int d, std::vector<float> data_, int block_ = BLOCK_SIZE)
+        : num_nodes(n), dim(d), block(block_), data(std::move(data_)), adj((size_t)n * block_, -1) {}
+
+    std::vector<int> search(const float* query, int k, int ef_search = 0, QueryTrace* trace = nullptr) const {
+        std::vector<int> ids(k);
+        QueryTrace t;
+        check(song_cpu_search(data.data(), adj.data(), num_nodes, dim, block, query, 1, k, ef_search, N_MULTIPROBE,
+                              1, ids.data(), nullptr, &t, nullptr, nullptr),
+              "song_cpu_search");
+        ids.resize(t.result_count);
+        if (trace) *trace = t;
+        return ids;
+    }
+};
+
+// build_graph (graph.h:190-199): the same NSW build as build_kernel_graph.
+inline Graph build_graph(std::vector<float> data, int n, int dim, const BuildConfig& bc = {}) {
+    Graph g(n, dim, std::move(data), bc.block);
+    check(song_nsw_build(g.data.data(), n, dim, SONG_METRIC_L2, bc.block, bc.fixed_degree, bc.search_degree,
+                         bc.ef_construction, bc.n_threads, bc.batch_div, bc.seq_prefix, bc.exact_order ? 1 : 0,
+                         bc.verbose ? 1 : 0, g.adj.data()),
+          "song_nsw_build");
+    return g;
+}
+
 // gpu_search (kernalGraph.h:307-382)
 inline std::vector<int> gpu_search(const KernelGraph& g, const float* queries, int nq, int k,
                                    int n_multiquery = N_MULTIQUERY, int ef_search = pq_size,

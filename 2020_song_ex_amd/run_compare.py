@@ -4,10 +4,11 @@ minus the official-SONG comparison and the plots).
 Same command line as the reference (run_compare.py:1191-1217): it prepares the
 base / query subsets (fvecs, cosine -> L2 normalisation), computes ground truth
 (here by GPU brute force over the resident vectors instead of numpy,
-run_compare.py:199-213), runs ``bench_gpu`` as a subprocess with the reference's
-flags (run_compare.py:1333-1350), and writes metrics_long.csv / metrics.csv /
-report.md from the bench's sweep_summary.csv (the same column schema the
-reference loads, run_compare.py:763-803).
+run_compare.py:199-213), runs ``bench_cpu`` (method repro_cpu_raw, unless
+--paper-only; run_compare.py:1316-1331) and ``bench_gpu`` (run_compare.py:1333-1350)
+as subprocesses with the reference's flags, and writes metrics_long.csv /
+metrics.csv / report.md from the benches' sweep_summary.csv files (the same column
+schema the reference loads, run_compare.py:763-803).
 
     python -m 2020_song_ex_amd.run_compare --base B.fvecs --query Q.fvecs --n 100000 --nq 1000
 """
@@ -105,7 +106,8 @@ def load_sweep_rows(path, k):
                    "qps": float(raw["qps_mean"]), "qps_std": float(raw["qps_std"]), "recall": float(raw["recall"]),
                    "k": k}
             for key in ("upload_ms", "query_projection_ms", "alloc_ms", "h2d_ms", "memset_ms", "kernel_ms",
-                        "d2h_ms", "visited_mean", "expanded_mean", "distance_evals_mean", "early_stop_rate"):
+                        "d2h_ms", "visited_mean", "expanded_mean", "distance_evals_mean", "early_stop_rate",
+                        "latency_mean_ms", "latency_p50_ms", "latency_p99_ms"):
                 row[key] = float(raw.get(key, 0) or 0)
             rows.append(row)
     if not rows:
@@ -142,6 +144,8 @@ def main(argv=None):
     ap.add_argument("--from-metrics", default="")
     ap.add_argument("--threads", type=int, default=0, help="NSW build threads (addition)")
     ap.add_argument("--batch-div", type=int, default=0, help="batched NSW build (addition; 0 = reference)")
+    ap.add_argument("--cpu-threads", type=int, default=1,
+                    help="search threads of the CPU method (addition; 1 = the reference's serial loop)")
     args = ap.parse_args(argv)
 
     preset = DATASET_PRESETS.get(args.dataset, {})
@@ -151,7 +155,8 @@ def main(argv=None):
     args.dist = args.dist or preset.get("dist", "l2")
     out_dir = Path(args.out_dir).resolve()
     prepared, runs = out_dir / "prepared", out_dir / "runs" / "repro_gpu"
-    for d in (prepared, runs):
+    runs_cpu = out_dir / "runs" / "repro_cpu_raw"
+    for d in (prepared, runs, runs_cpu):
         d.mkdir(parents=True, exist_ok=True)
     if args.from_metrics:
         rows = list(csv.DictReader(open(args.from_metrics)))
@@ -181,7 +186,23 @@ def main(argv=None):
         np.save(prepared / "base.npy", base)
         np.save(prepared / "query.npy", query)
 
-    print("[2/4] running repro GPU raw/hash sweep...", flush=True)
+    threads = args.threads or min(16, os.cpu_count() or 1)
+    steps = 4 if args.paper_only else 5
+    step = 2
+    if not args.paper_only:
+        print(f"[{step}/{steps}] running repro CPU raw sweep...", flush=True)
+        step += 1
+        cpu_exe = HERE / "bench_cpu"
+        if not cpu_exe.exists():
+            raise RuntimeError(f"{cpu_exe} is not built (make -C 2020_song_ex_amd/csrc)")
+        run([str(cpu_exe), "--base", str(bench_base), "--query", str(bench_query), "--n", str(base.shape[0]),
+             "--nq", str(query.shape[0]), "--k", str(args.k), "--pq-list", args.pq_list,
+             "--repeats", str(args.repeats), "--gt-npy", str(prepared / "gt.npy"), "--out-dir", str(runs_cpu),
+             "--ids-out", str(out_dir / "repro_cpu_raw.ids"), "--json-out", str(out_dir / "repro_cpu_raw.json"),
+             "--threads", str(args.cpu_threads), "--build-threads", str(threads),
+             "--batch-div", str(args.batch_div)])
+    print(f"[{step}/{steps}] running repro GPU raw/hash sweep...", flush=True)
+    step += 1
     exe = HERE / "bench_gpu"
     if not exe.exists():
         raise RuntimeError(f"{exe} is not built (make -C 2020_song_ex_amd/csrc)")
@@ -190,22 +211,27 @@ def main(argv=None):
            "--pq-list", args.pq_list, "--repeats", str(args.repeats), "--batch-size", str(args.batch_size),
            "--out-dir", str(runs), "--raw-ids-out", str(out_dir / "repro_gpu_raw.ids"),
            "--hash-ids-out", str(out_dir / "repro_gpu_hash.ids"), "--json-out", str(out_dir / "repro_gpu.json"),
-           "--threads", str(args.threads or min(16, os.cpu_count() or 1)), "--batch-div", str(args.batch_div)]
+           "--threads", str(threads), "--batch-div", str(args.batch_div)]
     if args.paper_only:
         cmd += ["--no-hash", "1"]
     run(cmd)
 
-    print("[3/4] evaluating...", flush=True)
-    rows = load_sweep_rows(runs / "sweep_summary.csv", args.k)
+    print(f"[{step}/{steps}] evaluating...", flush=True)
+    step += 1
+    rows = [] if args.paper_only else load_sweep_rows(runs_cpu / "sweep_summary.csv", args.k)
+    rows += load_sweep_rows(runs / "sweep_summary.csv", args.k)
     for r in rows:  # recall recomputed from the ids files, as the reference does for official runs
-        ids_path = runs / f"{r['method']}_pq{r['pq']:04d}.ids"
+        if r["method"] == "repro_cpu_raw":
+            ids_path = runs_cpu / f"pq{r['pq']:04d}.ids"
+        else:
+            ids_path = runs / f"{r['method']}_pq{r['pq']:04d}.ids"
         if ids_path.exists():
             r["recall_from_ids"] = recall_at_k(read_ids(ids_path, query.shape[0], args.k), gt, args.k)
     write_csv(out_dir / "metrics_long.csv", rows)
     primary = max(r["pq"] for r in rows)
     prim = [r for r in rows if r["pq"] == primary]
     write_csv(out_dir / "metrics.csv", prim, ["method", "build_ms", "search_ms", "qps", "recall", "k"])
-    print("[4/4] report...", flush=True)
+    print(f"[{step}/{steps}] report...", flush=True)
     write_report(out_dir / "report.md", prim, args)
     (out_dir / "manifest.json").write_text(json.dumps({"subset": {"n": args.n, "nq": args.nq, "k": args.k},
                                                        "metrics_long_csv": str(out_dir / "metrics_long.csv")},

@@ -21,7 +21,7 @@ def traces():
     return {k: z[k] for k in z.files}
 
 
-@pytest.mark.parametrize("name", sorted(traces()))
+@pytest.mark.parametrize("name", sorted(n for n in traces() if n.startswith("repro_gpu")))
 def test_reference_trace_relations(name):
     tr = traces()[name]
     block = 32  # BLOCK_SIZE, config.h:9
@@ -33,6 +33,30 @@ def test_reference_trace_relations(name):
     assert (tr[:, 0] == 4).all()                             # four seeds
     assert (tr[:, 8] <= pq).all() and (tr[:, 9] <= pq).all()  # peaks bounded by the queue
     assert (tr[:, 10] == 10).all()
+
+
+@pytest.mark.parametrize("pq", [10, 30, 50])
+def test_reference_cpu_trace_relations_hold_for_the_cpu_search(song, datasets, pq):
+    """The reference's own bench_cpu traces (Graph::search on SIFT1M) and song_cpu_search's
+    traces on a SIFT-like graph keep the same relations: every fresh node enters the unbounded
+    frontier, the stopping pop scans no row, the result set is bounded by the queue."""
+    def check(tr, block, pq):
+        assert (tr[:, 1] == tr[:, 4] + tr[:, 0]).all()
+        assert (tr[:, 5] == tr[:, 1]).all()
+        assert (tr[:, 6] == tr[:, 1]).all()                       # frontier pushes = visited
+        assert (tr[:, 3] == block * (tr[:, 2] - tr[:, 11])).all()
+        assert (tr[:, 9] <= pq).all() and (tr[:, 7] >= tr[:, 9]).all()
+        assert (tr[:, 10] == 10).all()
+    ref = traces()[f"repro_cpu_raw_pq{pq:04d}"]
+    assert ref.shape == (1000, 12) and (ref[:, 0] == 4).all()
+    check(ref, 32, pq)
+    base, q = datasets.gmm_int(20000, 100, dim=128, n_centres=100, sigma=25.0, seed=2)
+    adj = song.nsw_build(base, block=32, n_threads=4, batch_div=64, seq_prefix=1024, exact_order=False)
+    _, _, tr, _, _ = song.cpu_search(base, adj, q, 10, pq)
+    check(tr, 32, pq)
+    # work per expansion of the same order as the reference's run
+    mine, theirs = tr[:, 4].sum() / tr[:, 2].sum(), ref[:, 4].sum() / ref[:, 2].sum()
+    assert 0.3 * theirs < mine < 3 * theirs, (mine, theirs)
 
 
 def test_oracle_traces_keep_the_same_relations_at_reference_scale(oracle, song, datasets):

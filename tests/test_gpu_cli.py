@@ -1,6 +1,7 @@
 """The reference's command-line boundary end to end on the device: the
-run_compare driver (run_compare.py:1191-1418) runs bench_gpu as a subprocess
-with the reference's flags, and both write the reference's files."""
+run_compare driver (run_compare.py:1191-1418) runs bench_cpu and bench_gpu as
+subprocesses with the reference's flags, and all of them write the reference's
+files."""
 import csv
 import importlib
 import json
@@ -34,7 +35,7 @@ def compare_run(tmp_path_factory, datasets):
 def test_driver_outputs(compare_run):
     base, q, out = compare_run
     rows = list(csv.DictReader(open(out / "metrics_long.csv")))
-    assert {r["method"] for r in rows} == {"repro_gpu_raw", "repro_gpu_hash"}
+    assert {r["method"] for r in rows} == {"repro_cpu_raw", "repro_gpu_raw", "repro_gpu_hash"}
     assert {int(r["pq"]) for r in rows} == {16, 64}
     for r in rows:
         # the summary's recall is the recall of the ids file it names
@@ -42,6 +43,9 @@ def test_driver_outputs(compare_run):
         assert float(r["qps"]) > 0
     raw = {int(r["pq"]): float(r["recall"]) for r in rows if r["method"] == "repro_gpu_raw"}
     assert raw[64] > raw[16] > 0.1  # recall rises with the queue
+    cpu = {int(r["pq"]): float(r["recall"]) for r in rows if r["method"] == "repro_cpu_raw"}
+    assert cpu[64] >= cpu[16] > 0.1
+    assert (out / "repro_cpu_raw.ids").exists() and (out / "repro_cpu_raw.json").exists()
     assert (out / "metrics.csv").exists() and (out / "report.md").exists()
 
 
@@ -74,6 +78,16 @@ def test_raw_ids_equal_the_library_search(song, compare_run):
     finally:
         idx.close()
     np.testing.assert_array_equal(got, r.ids)
+
+
+def test_cpu_ids_equal_the_library_cpu_search(song, compare_run):
+    """The driver's CPU method is song_cpu_search (Graph::search) on the same reference-built
+    graph, its ids worst first as the reference's bench_cpu writes them."""
+    base, q, out = compare_run
+    got = np.array([[int(t) for t in line.split()] for line in open(out / "repro_cpu_raw.ids")])
+    adj = song.nsw_build(base, block=32, n_threads=4)
+    ids, _, _, _, _ = song.cpu_search(base, adj, q, 10, 64, best_first=False)
+    np.testing.assert_array_equal(got, ids)
 
 
 def test_hash_ids_equal_the_library_search(song, compare_run):
