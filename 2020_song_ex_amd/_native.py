@@ -53,7 +53,7 @@ EXPORTS = (
     "song_learn_trace", "song_mlp_train", "song_mlp_forward", "song_server_create", "song_server_submit",
     "song_server_poll", "song_server_wait", "song_server_stats_get", "song_server_destroy",
     "song_server_run_poisson", "song_search_device_static",
-    "song_cpu_search",
+    "song_cpu_search", "song_dev_gather",
 )
 
 _lib = None
@@ -99,6 +99,7 @@ def lib():
     L.song_queries_to_device.argtypes = [_P, _P, _I32, ctypes.POINTER(_P)]
     L.song_index_bruteforce.argtypes = [_P, _P, _I32, _I32, _P, _P, _P]
     L.song_dev_copy.argtypes = [_P, _P, _I64, _P]
+    L.song_dev_gather.argtypes = [_P, _P, _I64, _I32, _P, _I64, _P]
     L.song_search_device_mixed.argtypes = [_P, _P, _I32, _P, _P, _I32, _I32, _P, _P, _P, _P, _P]
     _lib = L
     return L

@@ -108,6 +108,19 @@ __global__ void copy16_kernel(const uint4* __restrict__ src, uint4* __restrict__
         dst[i] = src[i];
 }
 
+// Rows gathered at given ids, 16 bytes per lane: dst row r = src row ids[r].
+// Its traffic is known exactly (rows of row_bytes at scattered ids, as the
+// search reads them), which calibrates hardware byte counters on the search's
+// access pattern rather than on a stream.
+__global__ void gather16_kernel(const uint4* __restrict__ src, const int32_t* __restrict__ ids,
+                                uint4* __restrict__ dst, int64_t n_rows, int chunks) {
+    const int64_t total = n_rows * chunks;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = i / chunks;
+        dst[i] = src[(int64_t)ids[r] * chunks + (i - r * chunks)];
+    }
+}
+
 size_t row_bytes(const song_index* ix) { return (size_t)ix->dstride * (ix->metric == SONG_METRIC_HAMMING ? 8 : 4); }
 size_t host_row_bytes(const song_index* ix) {
     return ix->metric == SONG_METRIC_HAMMING ? 32 : (size_t)ix->dim * 4;
@@ -1139,6 +1152,38 @@ int song_dev_copy(void* dst, const void* src, int64_t bytes, double* kernel_ms_o
     HIP_TRY(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
+    if (kernel_ms_out) *kernel_ms_out = ms;
+    return SONG_OK;
+}
+
+int song_dev_gather(void* dst, const void* src, int64_t src_rows, int32_t row_bytes, const int32_t* ids,
+                    int64_t n_ids, double* kernel_ms_out) {
+    if (!dst || !src || !ids || n_ids < 1 || src_rows < 1) return fail(SONG_ERR_ARG, "bad gather arguments");
+    if (row_bytes < 16 || row_bytes % 16 != 0) return fail(SONG_ERR_ARG, "row_bytes must be a positive multiple of 16");
+    for (int64_t i = 0; i < n_ids; i++)
+        if (ids[i] < 0 || ids[i] >= src_rows) return fail(SONG_ERR_ARG, "gather id out of range");
+    int32_t* d_ids = nullptr;
+    HIP_TRY(hipMalloc(&d_ids, (size_t)n_ids * 4), "hipMalloc gather ids");
+    hipEvent_t a = nullptr, b = nullptr;
+    hipError_t e = hipMemcpy(d_ids, ids, (size_t)n_ids * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipEventCreate(&a);
+    if (e == hipSuccess) e = hipEventCreate(&b);
+    const int chunks = row_bytes / 16;
+    const int blocks = (int)std::min<int64_t>(8192, std::max<int64_t>(1, (n_ids * chunks + 255) / 256));
+    if (e == hipSuccess) e = hipEventRecord(a, 0);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(gather16_kernel, dim3(blocks), dim3(256), 0, 0, (const uint4*)src, d_ids, (uint4*)dst,
+                           n_ids, chunks);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(b, 0);
+    if (e == hipSuccess) e = hipEventSynchronize(b);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    (void)hipFree(d_ids);
+    if (e != hipSuccess) return hip_fail(e, "gather kernel");
     if (kernel_ms_out) *kernel_ms_out = ms;
     return SONG_OK;
 }

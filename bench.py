@@ -17,14 +17,26 @@ the mixture and then timed on the bench queries:
     stop network (and, where the tuner picks them, the cost-ordered queue and
     the hop cap), networks trained on 15,000 held-out draws.
 value = queries per second of the learned + persistent method when its
-recall@10 on the bench queries reaches the target, else of the plain method;
-both are in the JSON line with their ratio.  The line also carries the
-roofline of the search kernel behind `value` (algorithmic bytes from the run's
-own trace counters / HIP-event kernel time), Poisson-load serving latency (p50 /
-p99) and the reference's CPU search (Graph::search, graph.h:33-115, compiled
-from its sources into oracle/_ref) timed on this host on a bounded sample with
-one thread (the reference's own setting) and on all queries split over the
-process's cores (cpu_baseline.all_cores), with the CPU model.
+recall@10 on the bench queries reaches the target, else of the plain method.
+Beside each method's device-resident rate the line carries its rate through
+host buffers (song_search: query H2D + kernel + ids D2H per call, the QPS the
+reference's gpu_search reports, kernalGraph.h:341-376).
+
+The line also carries:
+  * roofline -- algorithmic bytes of the kernel behind `value` (from the run's
+    own trace counters) / its HIP-event launch time, beside the rocprofv3
+    average of the same kernel and workload when profiles/ holds one;
+  * latency -- the continuous-batching server under Poisson arrivals at 0.5,
+    0.7 and 0.9 of its own measured saturation rate, each load running about
+    1.5 s of arrivals (the query set replayed), the first fifth dropped as
+    warm-up; p50 / p99 per request, achieved beside offered;
+  * cpu_baseline -- the reference's own Graph::search (graph.h:33-115,
+    compiled from its sources into oracle/_ref) on this host: one thread on a
+    bounded sample at the plain method's queue, the library's song_cpu_search
+    (the same ids, checked) over all queries on the host's CPU share, and the
+    queue at which the CPU search reaches the GPU method's recall;
+  * config1 -- the small CPU-runnable workload (10,000 x 128, 100 queries,
+    out-degree 16, queue 64, 2^15-bit filter) on the GPU and the CPU.
 """
 import os
 
@@ -74,12 +86,59 @@ def cpu_model():
     return "unknown"
 
 
+def host_cpu_share():
+    """Host threads this process may use: its affinity set, bounded by a cgroup CPU quota when
+    one is visible, else by 16 (a one-GPU box's share of its host)."""
+    nproc = os.cpu_count() or 1
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = nproc
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()[:2]
+            if q != "max":
+                quota = max(1, int(int(q) // int(period)))
+    except (OSError, ValueError):
+        pass
+    threads = min(affinity, quota) if quota else min(affinity, 16)
+    return {"nproc": nproc, "affinity": affinity, "cgroup_quota_cpus": quota, "threads": threads,
+            "rule": "cgroup quota" if quota else "affinity capped at the one-GPU share of 16"}
+
+
 def algorithmic_bytes(trace, dim, block, k):
     """Sum over queries of distance_evals*row + expanded*row_of_adjacency + query + ids."""
     devals = trace[:, 5].astype(np.int64).sum()
     expanded = trace[:, 2].astype(np.int64).sum()
     nq = trace.shape[0]
     return int(devals * 4 * dim + expanded * 4 * block + nq * (4 * dim + 4 * k))
+
+
+def calibration(N, args):
+    """Byte-counter calibration kernels of known traffic (run under rocprofv3 --pmc)."""
+    import ctypes
+    ms = ctypes.c_double()
+    if args.copy_calibration:
+        nbytes = args.copy_calibration << 20
+        a, b = N.DeviceBuffer(nbytes), N.DeviceBuffer(nbytes)
+        for _ in range(3):
+            N.check(N.lib().song_dev_copy(b.ptr, a.ptr, nbytes, ctypes.byref(ms)), "song_dev_copy")
+        print(json.dumps({"kernel": "copy16_kernel", "bytes_read": nbytes, "bytes_written": nbytes,
+                          "kernel_ms": ms.value, "GB_per_s": 2 * nbytes / (ms.value / 1e3) / 1e9}))
+    else:
+        row = 512  # one 128-float vector row, as the search gathers them
+        src_rows = 1 << 21  # 1 GiB of rows, four times the last-level cache
+        n_ids = (args.gather_calibration << 20) // row
+        rng = np.random.default_rng(5)
+        ids = rng.integers(0, src_rows, size=n_ids).astype(np.int32)
+        src, dst = N.DeviceBuffer(src_rows * row), N.DeviceBuffer(n_ids * row)
+        for _ in range(3):
+            N.check(N.lib().song_dev_gather(dst.ptr, src.ptr, src_rows, row, N.ptr(ids), n_ids, ctypes.byref(ms)),
+                    "song_dev_gather")
+        print(json.dumps({"kernel": "gather16_kernel", "rows": n_ids, "row_bytes": row,
+                          "bytes_read": n_ids * row + n_ids * 4, "bytes_written": n_ids * row,
+                          "kernel_ms": ms.value, "GB_per_s": (2 * n_ids * row + 4 * n_ids) / (ms.value / 1e3) / 1e9}))
 
 
 def main():
@@ -92,11 +151,14 @@ def main():
     ap.add_argument("--k", type=int, default=10)
     ap.add_argument("--ef", type=int, default=0, help="plain queue size (0: smallest with recall >= target)")
     ap.add_argument("--target-recall", type=float, default=0.90)
-    ap.add_argument("--threads", type=int, default=0)
+    ap.add_argument("--threads", type=int, default=0, help="host threads (0: this process's CPU share)")
     ap.add_argument("--graph", default="", help="load/save the graph file here")
-    ap.add_argument("--cpu-sample", type=int, default=4000, help="queries timed on the CPU baseline")
+    ap.add_argument("--exact-graph", action="store_true",
+                    help="build the reference's sequential NSW graph (build_graph) instead of the batched one")
+    ap.add_argument("--cpu-sample", type=int, default=4000, help="queries timed on one CPU thread")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-config1", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--model", default="", help="load the learned model from this file if it exists, "
                                                  "else train it and save it there")
@@ -109,12 +171,16 @@ def main():
     ap.add_argument("--train", type=int, default=15000, help="training queries for the learned search")
     ap.add_argument("--valid", type=int, default=10000, help="validation queries for the autotuner")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--serve-loads", default="0.1,0.2,0.3",
-                    help="Poisson rates as fractions of the batch QPS (the launch-slot server saturates near a quarter of it); the middle one is the headline")
+    ap.add_argument("--serve-loads", default="0.5,0.7,0.9",
+                    help="Poisson rates as fractions of the server's measured saturation rate; the middle one "
+                         "is the headline")
+    ap.add_argument("--serve-seconds", type=float, default=1.5, help="arrival span of each load")
     ap.add_argument("--serve-batch", type=int, default=256)
     ap.add_argument("--serve-streams", type=int, default=16)
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run a plain copy kernel of this many MiB (byte-counter calibration)")
+    ap.add_argument("--gather-calibration", type=int, default=0,
+                    help="only run a 512-byte row gather at random ids, this many MiB (byte-counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -125,17 +191,11 @@ def main():
     song = importlib.import_module("2020_song_ex_amd")
     datasets = importlib.import_module("2020_song_ex_amd.datasets")
     N = song._native
-    if args.copy_calibration:
-        import ctypes
-        nbytes = args.copy_calibration << 20
-        a, b = N.DeviceBuffer(nbytes), N.DeviceBuffer(nbytes)
-        ms = ctypes.c_double()
-        for _ in range(3):
-            N.check(N.lib().song_dev_copy(b.ptr, a.ptr, nbytes, ctypes.byref(ms)), "song_dev_copy")
-        print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "kernel_ms": ms.value,
-                          "GB_per_s": 2 * nbytes / (ms.value / 1e3) / 1e9}))
+    if args.copy_calibration or args.gather_calibration:
+        calibration(N, args)
         return
-    threads = args.threads or min(16, os.cpu_count() or 1)
+    share = host_cpu_share()
+    threads = args.threads or share["threads"]
 
     t0 = time.time()
     n_extra = 0 if (args.no_learned and args.ef) else args.train + args.valid
@@ -148,11 +208,15 @@ def main():
     if args.graph and os.path.exists(args.graph):
         adj, _ = song.graph_load(args.graph)
         log(f"graph loaded from {args.graph}")
+    graph_kind = "exact sequential (build_graph)" if args.exact_graph else "batched (batch_div 64 after 1024)"
     if adj is None:
         t0 = time.time()
-        adj = song.nsw_build(base, block=block, n_threads=threads, batch_div=64, seq_prefix=1024,
-                             exact_order=False, verbose=True)
-        log(f"NSW graph built in {time.time() - t0:.1f}s ({threads} threads, batched)")
+        if args.exact_graph:
+            adj = song.nsw_build(base, block=block, exact_order=True, verbose=True)
+        else:
+            adj = song.nsw_build(base, block=block, n_threads=threads, batch_div=64, seq_prefix=1024,
+                                 exact_order=False, verbose=True)
+        log(f"NSW graph built in {time.time() - t0:.1f}s ({graph_kind}, {threads} threads)")
         if args.graph:
             song.graph_save(args.graph, adj, dim=dim)
 
@@ -187,6 +251,20 @@ def main():
         wall = time.perf_counter() - t0
         return nq * args.steps / wall, wall * 1e3 / args.steps, float(np.mean(kms)), float(np.std(kms))
 
+    def host_buffers(k, ef, reps=5, **kw):
+        """Through song_search with host buffers: nq / the call's own batch-loop time (query H2D,
+        kernel, ids D2H; allocation excluded), as the reference's gpu_search reports it.
+        Returns (mean QPS, population std, mean h2d / kernel / d2h ms)."""
+        idx.search(queries, k, ef, dists=False, trace=False, **kw)  # sizes the host-path buffers
+        qps, parts = [], []
+        for _ in range(reps):
+            r = idx.search(queries, k, ef, dists=False, trace=False, **kw)
+            qps.append(nq / (r.timing["total_ms"] / 1e3))
+            parts.append((r.timing["h2d_ms"], r.timing["kernel_ms"], r.timing["d2h_ms"]))
+        p = np.mean(parts, axis=0)
+        return {"qps": round(float(np.mean(qps)), 1), "qps_std": round(float(np.std(qps)), 1),
+                "h2d_ms": round(float(p[0]), 4), "kernel_ms": round(float(p[1]), 4), "d2h_ms": round(float(p[2]), 4)}
+
     sweep = []
     efs = (32, 64, 128, 256, 512)
     if not args.no_sweep:
@@ -213,6 +291,8 @@ def main():
     # rocprofv3 records describe) the same kernels; --retune re-runs the tuner.
     tuned_path = os.path.join(ROOT, "profiles", "tuned_configs.json")
     wkey = f"{args.n}x{dim} nq={nq} k={args.k} recall>={args.target_recall}"
+    if args.exact_graph:
+        wkey += " graph=exact"
     tuned_store = json.load(open(tuned_path)) if os.path.exists(tuned_path) else {}
     stored = {} if args.retune else tuned_store.get(wkey, {})
 
@@ -248,6 +328,8 @@ def main():
     plain_rec = recall_at(ids, gt, 10)
     plain_qps, plain_ms_step, plain_kernel_ms, plain_kernel_std = timed(args.k, ef, **plain_params)
     log(f"plain: queue {ef} {plain_params}, recall@10 {plain_rec:.4f}, {plain_qps:.0f} queries/s")
+    plain_host = host_buffers(args.k, ef, **plain_params)
+    log(f"plain through host buffers (query H2D + kernel + ids D2H): {plain_host}")
 
     def slots_for(q):
         return (1 if q <= 64 else 2 if q <= 128 else 3 if q <= 192 else 4 if q <= 256 else 6 if q <= 384
@@ -263,32 +345,31 @@ def main():
         chunks *= 2
 
     def roofline_of(trace_rows, kernel_ms, kname, key):
-        """Algorithmic bytes of one launch / its HIP-event duration; HBM traffic per launch
-        from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE record of this exact kernel and
-        workload, when there is one."""
+        """Algorithmic bytes of one launch / its HIP-event duration.  Beside it, from the committed
+        rocprofv3 records of this exact kernel and workload when there are some: the kernel's
+        average duration and the fraction it gives, and the HBM traffic per launch
+        (FETCH_SIZE / WRITE_SIZE scaled by their calibration)."""
         abytes = algorithmic_bytes(trace_rows, dim, block, args.k)
         achieved = abytes / (kernel_ms / 1e3) / 1e9
-        traffic = None
-        tpath = os.path.join(ROOT, "profiles", "traffic.json")
-        if os.path.exists(tpath):
-            trec = json.load(open(tpath)).get(key)
-            if trec:
-                traffic = trec["bytes_per_launch"]
-        return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kname,
-                "kernel_avg_ms": round(kernel_ms, 4), "algorithmic_bytes_per_launch": abytes}
+        out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+               "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kname,
+               "kernel_avg_ms": round(kernel_ms, 4), "algorithmic_bytes_per_launch": abytes}
+        for fname, field in (("traffic.json", "traffic"), ("kernel_profiles.json", "rocprof")):
+            path = os.path.join(ROOT, "profiles", fname)
+            rec = json.load(open(path)).get(key) if os.path.exists(path) else None
+            if rec and field == "traffic":
+                out["traffic"] = rec["bytes_per_launch"]
+                out["traffic_summary"] = rec["summary"]
+            elif rec:
+                out["rocprof_avg_ms"] = rec["avg_ms"]
+                out["rocprof_frac"] = round(abytes / (rec["avg_ms"] / 1e3) / 1e9 / HBM_PEAK_GBS, 4)
+                out["rocprof_summary"] = rec["summary"]
+        return out
 
     pname = f"search_kernel<0,{slots_for(ef)},{lanes},{chunks},0>"
     plain_roofline = roofline_of(trace, plain_kernel_ms, pname,
                                  f"{pname} {args.n}x{dim} nq={nq} queue={ef} k={args.k} "
                                  f"bloom_bits={plain_params.get('bloom_bits', 1 << 14)}")
-
-    # ---------------- the host-buffer path (song_search: queries H2D, ids D2H), for the record
-    t0 = time.perf_counter()
-    for _ in range(3):
-        idx.search(queries, args.k, ef, dists=False, trace=False, **plain_params)
-    pcie_qps = 3 * nq / (time.perf_counter() - t0)
-    log(f"plain through host buffers (PCIe included): {pcie_qps:.0f} queries/s")
 
     # ---------------- persistent kernel alone (no learning), for the record
     pers_qps, _, pers_kernel_ms, _ = timed(args.k, ef, mode=1)
@@ -297,7 +378,6 @@ def main():
     # ---------------- learned + persistent
     learned_out = None
     learned_roofline = None
-    serve_out = None
     best_search = None
     if not args.no_learned:
         learned = importlib.import_module("2020_song_ex_amd.learned")
@@ -338,6 +418,7 @@ def main():
             ids, ltr, _ = run(args.k, lef, **p)
             lrec = recall_at(ids, gt, 10)
             lqps, lms_step, lkernel_ms, lkernel_std = timed(args.k, lef, **p)
+            lhost = host_buffers(args.k, lef, **p)
             best_search = (lef, p)
             lname = f"persistent_kernel<0,{slots_for(lef)},{lanes},{chunks},1>"
             lkey = f"{lname} {args.n}x{dim} nq={nq} " + " ".join(f"{kk}={v}" for kk, v in sorted({"ef": lef, **p}.items()))
@@ -345,6 +426,7 @@ def main():
             learned_out = {"params": {"ef": lef, **p}, "recall_at_10": round(lrec, 4), "qps": round(lqps, 1),
                            "ms_per_step": round(lms_step, 4), "kernel_ms": round(lkernel_ms, 4),
                            "kernel_ms_std": round(lkernel_std, 4), "qps_kernel": round(nq / (lkernel_ms / 1e3), 1),
+                           "host_buffers": lhost,
                            "expanded_mean": float(ltr[:, 2].mean()),
                            # the launch ends with its longest searches
                            "expanded_p50_p95_max": [float(np.percentile(ltr[:, 2], 50)),
@@ -352,70 +434,33 @@ def main():
                            "learned_stops": float((ltr[:, 11] == 2).mean()),
                            "hop_capped": float((ltr[:, 11] == 3).mean()),
                            "speedup_vs_plain": round(lqps / plain_qps, 3),
+                           "speedup_vs_plain_host_buffers": round(lhost["qps"] / plain_host["qps"], 3),
                            "validation_recall": round(res.best.recall, 4), "tuned": res.table()}
             log(f"learned+persistent: {learned_out['params']} recall@10 {lrec:.4f}, {lqps:.0f} queries/s "
-                f"({lqps / plain_qps:.2f}x plain)")
+                f"({lqps / plain_qps:.2f}x plain); host buffers {lhost}")
         else:
             learned_out = {"error": "no tuned configuration reached the target recall", "tuned": res.table()}
 
-    # ---------------- serving: Poisson arrivals, per-request latency
-    if not args.no_serve:
-        serving = importlib.import_module("2020_song_ex_amd.serving")
-        sef, sp = best_search if best_search else (ef, dict(plain_params))
-        # one wave per request in each launch (a persistent grid per small launch only adds waves)
-        sp = {**sp, "mode": 0}
-        cap_qps = learned_out["qps"] if best_search else plain_qps
-        ks = np.full(nq, args.k, np.int32)
-        efs_req = np.full(nq, sef, np.int32)
-        curve = []
-        for load in [float(x) for x in args.serve_loads.split(",")]:
-            rate = load * cap_qps
-            sv = serving.Server(idx, max_batch=args.serve_batch, n_streams=args.serve_streams, k_max=args.k,
-                                ef_max=512, **sp)
-            sids, _, lat, wall = sv.run_poisson(queries, ks, efs_req, rate, seed=11)
-            st = sv.stats()
-            sv.close()
-            point = {"load": load, "offered_qps": round(rate, 1), "achieved_qps": round(nq / (wall / 1e3), 1),
-                     "p50_ms": round(float(np.percentile(lat, 50)), 4),
-                     "p99_ms": round(float(np.percentile(lat, 99)), 4),
-                     "max_ms": round(float(lat.max()), 4), "recall_at_10": round(recall_at(sids, gt, 10), 4),
-                     "launches": st["launches"]}
-            curve.append(point)
-            log(f"serving {point}")
-        mid = curve[len(curve) // 2]
-        serve_out = {**mid, "queue": sef, "search": sp, "max_batch": args.serve_batch,
-                     "streams": args.serve_streams, "hw_queues": int(os.environ["GPU_MAX_HW_QUEUES"]),
-                     "curve": curve}
-
-    # ---------------- CPU baseline: the reference's own Graph::search
-    cpu = None
-    if not args.no_cpu:
-        oracle = importlib.import_module("oracle")
-        sample = min(args.cpu_sample, nq)
-        if oracle.ref_available():
-            cids, _, cms, _ = oracle.ref_graph_search(base, adj, queries[:sample], args.k, ef, n_threads=1)
-            kind = "reference"
-            cids = cids[:, ::-1]
-        else:
-            t0 = time.perf_counter()
-            cids, _ = oracle.cpu_search(base, adj, queries[:sample], args.k, ef, block=block)
-            cms = (time.perf_counter() - t0) * 1e3
-            kind = "port"
-        cpu = {"value": round(sample / (cms / 1e3), 1), "unit": "queries/s", "cores": 1, "kind": kind,
-               "sample": f"{sample} queries of the same workload, queue {ef}, k {args.k}, 1 thread "
-                         f"(Graph::search, graph.h:33-115)",
-               "recall_at_10": round(recall_at(cids, gt[:sample], 10), 4),
-               "cpu_model": cpu_model()}
-        if kind == "reference":
-            # the same search with the queries split over the host's cores (this process's share)
-            _, _, ams, _ = oracle.ref_graph_search(base, adj, queries, args.k, ef, n_threads=threads)
-            cpu["all_cores"] = {"value": round(nq / (ams / 1e3), 1), "cores": threads,
-                                "sample": f"all {nq} queries split over {threads} threads"}
-        log(f"cpu baseline {cpu}")
-
     use_learned = (learned_out is not None and "qps" in learned_out
                    and learned_out["recall_at_10"] >= args.target_recall)
-    # the roofline of the kernel behind `value`; the other method's in config
+
+    # ---------------- serving: Poisson arrivals, per-request latency
+    serve_out = None
+    if not args.no_serve:
+        sef, sp = best_search if best_search else (ef, dict(plain_params))
+        serve_out = serve_curve(idx, queries, gt, args.k, sef, {**sp, "mode": 0}, args)
+
+    # ---------------- CPU baseline: the reference's own Graph::search, and the library's restatement
+    cpu = None
+    gpu_recall = learned_out["recall_at_10"] if use_learned else plain_rec
+    if not args.no_cpu:
+        cpu = cpu_baseline(song, base, adj, queries, gt, args, ef, gpu_recall, threads, share)
+
+    # ---------------- the small CPU-runnable workload, GPU beside CPU
+    c1 = None
+    if not args.no_config1:
+        c1 = config1_leg(song, datasets, args, threads)
+
     roofline = learned_roofline if use_learned else plain_roofline
     value = learned_out["qps"] if use_learned else plain_qps
     ms_step = learned_out["ms_per_step"] if use_learned else plain_ms_step
@@ -426,18 +471,19 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"SONG search, {args.n}x{dim} integer GMM (1000 centres, sigma 25), "
                                f"{nq} queries, NSW out-degree {block}, L2, k {args.k}",
+                   "graph": graph_kind,
                    "method": "learned+persistent" if use_learned else "plain",
+                   "qps_host_buffers": (learned_out["host_buffers"]["qps"] if use_learned else plain_host["qps"]),
                    "plain": {"queue": ef, **plain_params, "recall_at_10": round(plain_rec, 4),
-                             "qps": round(plain_qps, 1), "kernel_ms": round(plain_kernel_ms, 4),
-                             "kernel_ms_std": round(plain_kernel_std, 4),
-                             "qps_kernel": round(nq / (plain_kernel_ms / 1e3), 1)},
-                   "plain_host_buffers_qps": round(pcie_qps, 1),
+                             "qps": round(plain_qps, 1), "ms_per_step": round(plain_ms_step, 4),
+                             "kernel_ms": round(plain_kernel_ms, 4), "kernel_ms_std": round(plain_kernel_std, 4),
+                             "qps_kernel": round(nq / (plain_kernel_ms / 1e3), 1), "host_buffers": plain_host},
                    "persistent_no_learning": {"queue": ef, "qps": round(pers_qps, 1),
                                               "kernel_ms": round(pers_kernel_ms, 4)},
                    "learned_vs_plain": (round(learned_out["qps"] / plain_qps, 3)
                                         if learned_out and "qps" in learned_out else None),
                    "plain_roofline": plain_roofline, "learned_roofline": learned_roofline,
-                   "learned_persistent": learned_out, "plain_tuned": plain_tuned},
+                   "learned_persistent": learned_out, "plain_tuned": plain_tuned, "config1": c1},
         "latency": serve_out,
         "roofline": roofline,
         "cpu_baseline": cpu,
@@ -445,6 +491,141 @@ def main():
     }
     print(json.dumps(out), flush=True)
     idx.close()
+
+
+def serve_curve(idx, queries, gt, k, sef, sp, args):
+    """The launch-slot server under Poisson arrivals.  First its saturation rate: arrivals offered
+    far faster than it serves, the served rate measured.  Then loads at fractions of that rate,
+    each offering about serve_seconds of arrivals (the query set replayed); the first fifth of
+    each load is warm-up and left out of the latency percentiles.  A load whose achieved rate
+    falls below 95 % of the offered rate is marked overloaded."""
+    serving = importlib.import_module("2020_song_ex_amd.serving")
+    nq = queries.shape[0]
+
+    def run_load(rate, n_req, seed):
+        reps = -(-n_req // nq)
+        qs = np.tile(queries, (reps, 1))[:n_req]
+        sv = serving.Server(idx, max_batch=args.serve_batch, n_streams=args.serve_streams, k_max=k, ef_max=512, **sp)
+        try:
+            ids, _, lat, wall = sv.run_poisson(qs, np.full(n_req, k, np.int32), np.full(n_req, sef, np.int32),
+                                               rate, seed=seed)
+            st = sv.stats()
+        finally:
+            sv.close()
+        return ids, lat, wall, st
+
+    n_sat = 20 * nq
+    _, _, wall, st = run_load(1e8, n_sat, 7)
+    sat = n_sat / (wall / 1e3)
+    log(f"serving saturation: {sat:.0f} requests/s ({n_sat} requests offered at once, {st['launches']} launches)")
+    curve = []
+    for load in [float(x) for x in args.serve_loads.split(",")]:
+        rate = load * sat
+        n_req = max(int(rate * args.serve_seconds), 2 * nq)
+        ids, lat, wall, st = run_load(rate, n_req, 11)
+        steady = lat[n_req // 5:]
+        achieved = n_req / (wall / 1e3)
+        point = {"load": load, "offered_qps": round(rate, 1), "achieved_qps": round(achieved, 1),
+                 "overloaded": bool(achieved < 0.95 * rate), "requests": n_req,
+                 "p50_ms": round(float(np.percentile(steady, 50)), 4),
+                 "p99_ms": round(float(np.percentile(steady, 99)), 4),
+                 "max_ms": round(float(steady.max()), 4),
+                 "recall_at_10": round(recall_at(ids[:nq], gt, min(10, k)), 4), "launches": st["launches"]}
+        curve.append(point)
+        log(f"serving {point}")
+    mid = curve[len(curve) // 2]
+    return {**mid, "saturation_qps": round(sat, 1), "queue": sef, "search": sp, "max_batch": args.serve_batch,
+            "streams": args.serve_streams, "hw_queues": int(os.environ["GPU_MAX_HW_QUEUES"]),
+            "warmup_dropped": "first 20 % of each load's requests", "curve": curve}
+
+
+def cpu_baseline(song, base, adj, queries, gt, args, ef, gpu_recall, threads, share):
+    """The reference's Graph::search compiled from its sources (oracle/_ref): one thread, the
+    reference's own setting, on a bounded sample at the plain method's queue.  The library's
+    song_cpu_search on the same sample (ids checked equal), and over all queries split across the
+    host share.  And the equal-recall point: the smallest queue at which the CPU search reaches
+    the GPU method's recall@10 on all queries, timed on one thread on the same sample."""
+    oracle = importlib.import_module("oracle")
+    nq = queries.shape[0]
+    sample = min(args.cpu_sample, nq)
+    k = args.k
+    out = {"unit": "queries/s", "cpu_model": cpu_model(), "host": share}
+    prod_ids, _, _, _, prod_ms = song.cpu_search(base, adj, queries[:sample], k, ef, n_threads=1, best_first=False)
+    if oracle.ref_available():
+        rids, _, rms, _ = oracle.ref_graph_search(base, adj, queries[:sample], k, ef, n_threads=1)
+        out.update({"value": round(sample / (rms / 1e3), 1), "cores": 1, "kind": "reference",
+                    "sample": f"{sample} queries of the same workload, queue {ef}, k {k}, 1 thread "
+                              f"(the reference's Graph::search, graph.h:33-115, built from its sources)",
+                    "recall_at_10": round(recall_at(rids[:, ::-1], gt[:sample], 10), 4),
+                    "product_ids_equal_reference": bool(np.array_equal(prod_ids, rids))})
+    else:
+        out.update({"value": round(sample / (prod_ms / 1e3), 1), "cores": 1, "kind": "port",
+                    "sample": f"{sample} queries, queue {ef}, k {k}, 1 thread (song_cpu_search)"})
+    out["product_1_thread"] = {"value": round(sample / (prod_ms / 1e3), 1), "sample": f"the same {sample} queries"}
+    all_ids, _, _, _, all_ms = song.cpu_search(base, adj, queries, k, ef, n_threads=threads)
+    out["all_cores"] = {"value": round(nq / (all_ms / 1e3), 1), "cores": threads,
+                        "sample": f"all {nq} queries split over {threads} threads (song_cpu_search)",
+                        "recall_at_10": round(recall_at(all_ids, gt, 10), 4)}
+    # equal recall: queues from small to large until the CPU reaches the GPU method's recall
+    eq = None
+    for q in (96, 128, 160, 192, 224, 256, 288, 320, 384, 448, 512):
+        ids, _, _, _, ms = song.cpu_search(base, adj, queries, k, q, n_threads=threads)
+        r = recall_at(ids, gt, 10)
+        if r >= gpu_recall:
+            eq = (q, r, ms)
+            break
+    if eq:
+        q, r, ms_all = eq
+        if oracle.ref_available():
+            _, _, rms, _ = oracle.ref_graph_search(base, adj, queries[:sample], k, q, n_threads=1)
+        else:
+            _, _, _, _, rms = song.cpu_search(base, adj, queries[:sample], k, q, n_threads=1)
+        out["equal_recall"] = {"queue": q, "recall_at_10": round(r, 4), "gpu_recall_at_10": round(gpu_recall, 4),
+                               "value_1_thread": round(sample / (rms / 1e3), 1),
+                               "value_all_cores": round(nq / (ms_all / 1e3), 1), "cores": threads}
+    log(f"cpu baseline {out}")
+    return out
+
+
+def config1_leg(song, datasets, args, threads):
+    """10,000 x 128 integer GMM (50 centres, sigma 20), 100 queries, out-degree 16 on the reference's
+    sequential build, L2, k 10, queue 64, 2^15-bit filter: the plain GPU kernel (device-resident and
+    through host buffers) beside the CPU search on one thread and on the host share."""
+    oracle = importlib.import_module("oracle")
+    c = datasets.config1()
+    base, q, block, k, ef, bb = c["base"], c["queries"], c["block"], c["k"], c["ef"], c["bloom_bits"]
+    t0 = time.time()
+    adj = song.nsw_build(base, block=block, exact_order=True)
+    build_s = time.time() - t0
+    nq = q.shape[0]
+    idx = song.Index(base, adj)
+    try:
+        gt, _, _ = idx.bruteforce(q, k)
+        r = idx.search(q, k, ef, bloom_bits=bb)
+        oi, od, ot = oracle.search(base, adj, q, k, ef, block=block, bloom_bits=bb, group=16)
+        parity = bool(np.array_equal(r.ids, oi) and np.array_equal(r.trace, ot))
+        N = song._native
+        dq = idx.queries_to_device(q)
+        d_ids = N.DeviceBuffer(nq * k * 4)
+        for _ in range(5):
+            idx.search_device(dq, nq, k, ef, d_ids, bloom_bits=bb)
+        kms = [idx.search_device(dq, nq, k, ef, d_ids, bloom_bits=bb)["kernel_ms"] for _ in range(50)]
+        tot = [idx.search(q, k, ef, bloom_bits=bb, dists=False, trace=False).timing["total_ms"] for _ in range(50)]
+    finally:
+        idx.close()
+    _, _, _, _, ms1 = song.cpu_search(base, adj, q, k, ef, n_threads=1)
+    cids, _, _, _, msn = song.cpu_search(base, adj, q, k, ef, n_threads=threads)
+    out = {"workload": f"10000x128 integer GMM (50 centres, sigma 20), {nq} queries, NSW out-degree {block} "
+                       f"(sequential build, {build_s:.1f} s), L2, k {k}, queue {ef}, Bloom 2^15",
+           "gpu_recall_at_10": round(recall_at(r.ids, gt, k), 4), "gpu_equals_oracle": parity,
+           "gpu_qps_kernel": round(nq / (float(np.median(kms)) / 1e3), 1),
+           "gpu_qps_host_buffers": round(nq / (float(np.median(tot)) / 1e3), 1),
+           "cpu_recall_at_10": round(recall_at(cids, gt, k), 4),
+           "cpu_qps_1_thread": round(nq / (ms1 / 1e3), 1),
+           "cpu_qps_all_cores": round(nq / (msn / 1e3), 1), "cpu_cores": threads,
+           "note": "100 queries fill under 1 % of the GPU: its time here is one launch's latency"}
+    log(f"config1 {out}")
+    return out
 
 
 if __name__ == "__main__":

@@ -443,6 +443,12 @@ int song_memcpy_h2d(void* dst, const void* src, int64_t bytes);
 /* Device-to-device copy by a plain streaming kernel (16 B per lane), so that
  * hardware byte counters can be calibrated against a known traffic. */
 int song_dev_copy(void* dst, const void* src, int64_t bytes, double* kernel_ms_out);
+/* Gathers n_ids rows of row_bytes (a multiple of 16) from the device buffer
+ * src [src_rows rows] into dst, dst row r = src row ids[r] (ids: a host array,
+ * each in [0, src_rows)), by a kernel reading 16 B per lane: a known traffic
+ * with the search's access pattern, for calibrating hardware byte counters. */
+int song_dev_gather(void* dst, const void* src, int64_t src_rows, int32_t row_bytes, const int32_t* ids,
+                    int64_t n_ids, double* kernel_ms_out);
 int song_memcpy_d2h(void* dst, const void* src, int64_t bytes);
 /* Copies nq host queries into a new device buffer in the index's row layout. */
 int song_queries_to_device(song_index* index, const void* queries, int32_t nq, void** d_out);
