@@ -78,6 +78,18 @@ def lib():
                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P,
                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P, _P]
         L.or_search_capped.restype = ctypes.c_int
+        L.or_search_learned.argtypes = [_P, _P, _I64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P,
+                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, _I64, ctypes.c_int,
+                                        ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P,
+                                        ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_int, _P, _P, _P]
+        L.or_search_learned.restype = ctypes.c_int
+        L.or_stop_logits.argtypes = [_P, ctypes.c_float, _P, ctypes.c_int, _P]
+        L.or_learned_trace.argtypes = [_P, _P, _I64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P,
+                                       ctypes.c_int, ctypes.c_int, ctypes.c_int, _I64, ctypes.c_int,
+                                       ctypes.c_int, _P, ctypes.c_int, ctypes.c_float, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_int, _P, _P, _P]
+        L.or_learned_trace.restype = ctypes.c_int
         L.or_cpu_search.argtypes = [_P, _P, _I64, ctypes.c_int, ctypes.c_int, _P, ctypes.c_int,
                                     ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P]
         L.or_cpu_search.restype = ctypes.c_int
@@ -153,6 +165,71 @@ def search(data, adj, queries, k, ef, *, block, metric="l2", bloom_bits=1 << 14,
     if rc != 0:
         raise ValueError("or_search rejected its arguments")
     return ids, d, tr
+
+
+def search_learned(data, adj, queries, k, ef, *, block, stop_w, dist_scale, threshold, min_hops, interval,
+                   confirm=1, max_hops=0, result_cap=0, metric="l2", bloom_bits=1 << 14, n_seeds=4, group=8,
+                   entries=None):
+    """The search with the learned stop network (the MI355X kernel's learned bit 1) re-executed on the
+    CPU: stop_w is the flat network (w1 [64 x 8], b1, w2 [64 x 64], b2, w3, then b3).
+    Returns (ids [nq, k] best-first, dists [nq, k], trace [nq, 12])."""
+    m = METRIC[metric]
+    if m == 2:
+        data = np.ascontiguousarray(data, dtype=np.uint64)
+        queries = np.ascontiguousarray(queries, dtype=np.uint64)
+        dim = 256
+    else:
+        data = np.ascontiguousarray(data, dtype=np.float32)
+        queries = np.ascontiguousarray(queries, dtype=np.float32)
+        dim = data.shape[1]
+    adj = np.ascontiguousarray(adj, dtype=np.int32)
+    w = np.ascontiguousarray(stop_w, dtype=np.float32)
+    n, nq = data.shape[0], queries.shape[0]
+    ids = np.empty((nq, k), np.int32)
+    d = np.empty((nq, k), np.float32)
+    tr = np.empty((nq, 12), np.int32)
+    ne = 0
+    if entries is not None:
+        entries = np.ascontiguousarray(entries, dtype=np.int32)
+        ne = entries.shape[1]
+    rc = lib().or_search_learned(_ptr(data), _ptr(adj), n, dim, block, m, _ptr(queries), nq, k, ef, bloom_bits,
+                                 n_seeds, group, _ptr(entries), ne, int(max_hops), int(result_cap), _ptr(w),
+                                 float(w[4800]), float(dist_scale), float(threshold), int(min_hops), int(interval),
+                                 int(confirm), _ptr(ids), _ptr(d), _ptr(tr))
+    if rc != 0:
+        raise ValueError("or_search_learned rejected its arguments")
+    return ids, d, tr
+
+
+def learned_trace(data, adj, queries, k, ef, *, block, dist_scale, min_hops, interval, max_evals,
+                  metric="l2", bloom_bits=1 << 14, n_seeds=4, group=8):
+    """The learned search's trace mode re-executed on the CPU: per consulted hop the 8 features
+    and the k-th best distance.  Returns (rows [nq, max_evals, 9], counts [nq], final_dk [nq])."""
+    m = METRIC[metric]
+    dt = np.uint64 if m == 2 else np.float32
+    data = np.ascontiguousarray(data, dtype=dt)
+    queries = np.ascontiguousarray(queries, dtype=dt)
+    dim = 256 if m == 2 else data.shape[1]
+    adj = np.ascontiguousarray(adj, dtype=np.int32)
+    nq = queries.shape[0]
+    rows = np.zeros((nq, max_evals, 9), np.float32)
+    cnt = np.empty(nq, np.int32)
+    fin = np.empty(nq, np.float32)
+    rc = lib().or_learned_trace(_ptr(data), _ptr(adj), data.shape[0], dim, block, m, _ptr(queries), nq, k, ef,
+                                bloom_bits, n_seeds, group, None, 0, float(dist_scale), int(min_hops), int(interval),
+                                int(max_evals), _ptr(rows), _ptr(cnt), _ptr(fin))
+    if rc != 0:
+        raise ValueError("or_learned_trace rejected its arguments")
+    return rows, cnt, fin
+
+
+def stop_logits(stop_w, features):
+    """The stop network's logits for feature rows [n, 8], in the kernel's operation order."""
+    w = np.ascontiguousarray(stop_w, dtype=np.float32)
+    f = np.ascontiguousarray(features, dtype=np.float32)
+    out = np.empty(f.shape[0], np.float32)
+    lib().or_stop_logits(_ptr(w), float(w[4800]), _ptr(f), f.shape[0], _ptr(out))
+    return out
 
 
 def cpu_search(data, adj, queries, k, ef, *, block, n_seeds=4):

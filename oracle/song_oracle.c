@@ -19,6 +19,9 @@
  *                              MI355X kernel uses, see or_row_dist below)
  *   - l2 (sequential sum)      distance.h:5-12
  *   - Graph::search            graph.h:33-115 (CPU baseline)
+ *   - the MI355X search's learned stop (no reference counterpart): its
+ *     per-hop features, consult schedule and network, as the kernel computes
+ *     them (or_search_learned, or_learned_trace, or_stop_logits)
  *   - brute-force ground truth bench_cpu.cpp:158-175, run_compare.py:199-213
  *
  * Everything is parameterised where the reference has compile-time
@@ -365,7 +368,82 @@ typedef struct {
     int res_cap;       /* result-set capacity: ef with the reference's early
                           stop; below ef (learned search's result_cap, no
                           reference counterpart) without that test           */
+    const struct or_stop *stop; /* learned stop network, or NULL            */
 } or_params;
+
+/* ------------------------------------------------------------------------- */
+/* The learned stop of the MI355X search (no reference counterpart)           */
+/* ------------------------------------------------------------------------- */
+
+/* The stop network and its consult schedule, as the search kernel applies
+ * them (song_search_impl.h, search_query's stop test and stop_logit):
+ * w: [64 x 8] w1 | [64] b1 | [64 x 64] w2 (row j = hidden unit j) | [64] b2 |
+ * [64] w3, then b3; features are consulted at hops min_hops (or interval when
+ * min_hops is 0), then every interval hops, once k distances are known. */
+typedef struct or_stop {
+    const float *w;
+    float b3, dist_scale, threshold;
+    int min_hops, interval, confirm;
+    /* recording instead of stopping (the kernel's trace mode): per consult
+     * the 8 features and the k-th best distance into rec [max_evals x 9] */
+    float *rec;
+    int max_evals, n_rec;
+} or_stop;
+
+/* The k smallest distances seen so far, ascending (k <= 64): a new distance
+ * d < the k-th smallest goes after the entries <= d; `inserts` counts them. */
+typedef struct {
+    float v[64];
+    int n, k, inserts;
+} or_topk;
+
+static float topk_dk(const or_topk *t) { return t->n >= t->k ? t->v[t->k - 1] : INFINITY; }
+
+static int topk_insert(or_topk *t, float d)
+{
+    if (!(d < topk_dk(t))) return 0;
+    int pos = 0;
+    while (pos < t->n && t->v[pos] <= d) pos++;
+    int last = t->n < t->k ? t->n : t->k - 1;
+    for (int i = last; i > pos; i--) t->v[i] = t->v[i - 1];
+    t->v[pos] = d;
+    if (t->n < t->k) t->n++;
+    t->inserts++;
+    return 1;
+}
+
+/* The network's logit in the kernel's operation order: each hidden unit an
+ * fmaf chain over its inputs in index order from its bias, ReLU, the output
+ * products w3[j] * h2[j] rounded on their own and summed as a pairwise tree
+ * over units (0+1, 2+3, ..., then pairs of pairs), plus b3. */
+static float stop_logit(const or_stop *st, const float f[8])
+{
+    const float *w1 = st->w, *b1 = w1 + 64 * 8, *w2 = b1 + 64, *b2 = w2 + 64 * 64, *w3 = b2 + 64;
+    float h1[64], t[64];
+    for (int j = 0; j < 64; j++) {
+        float h = b1[j];
+        for (int i = 0; i < 8; i++) h = fmaf(w1[j * 8 + i], f[i], h);
+        h1[j] = fmaxf(h, 0.f);
+    }
+    for (int j = 0; j < 64; j++) {
+        float a = b2[j];
+        for (int i = 0; i < 64; i++) a = fmaf(w2[j * 64 + i], h1[i], a);
+        a = fmaxf(a, 0.f);
+        t[j] = a * w3[j];
+    }
+    for (int w = 64; w > 1; w /= 2)
+        for (int j = 0; j < w / 2; j++) t[j] = t[2 * j] + t[2 * j + 1];
+    return t[0] + st->b3;
+}
+
+static float clamp8(float x) { return fminf(fmaxf(x, -8.f), 8.f); }
+
+/* Exposed for tests: the stop network's logit for n feature rows [n x 8]. */
+void or_stop_logits(const float *stop_w, float stop_b3, const float *features, int n, float *out)
+{
+    or_stop st = {stop_w, stop_b3, 1.f, 0.f, 0, 1, 1, NULL, 0, 0};
+    for (int i = 0; i < n; i++) out[i] = stop_logit(&st, features + (int64_t)i * 8);
+}
 
 /* Seeds at the entry of the search, kernalGraph.h:215-231: n_seeds evenly
  * spaced ids (size_t)p * n / n_seeds (N_MULTIPROBE = 4 in config.h:34). */
@@ -380,6 +458,8 @@ typedef struct {
     or_set cand, res;
     float *cd, *rd;
     int32_t *ci, *ri;
+    or_topk topk;        /* learned stop: the k smallest distances seen  */
+    int last_insert_hop; /* learned stop: hop of the last top-k insertion */
 } or_work;
 
 static int visited_test_set(const or_params *p, or_work *w, int id)
@@ -407,6 +487,9 @@ static void search_one(const or_space *sp, const or_params *p, or_work *w, const
                        int32_t *tr)
 {
     memset(tr, 0, TR_N * sizeof(int32_t));
+    memset(&w->topk, 0, sizeof w->topk);
+    w->topk.k = p->k;
+    w->last_insert_hop = 0;
     int frontier_cap = p->unbounded ? (int)(p->n + n_entry + 1) : p->ef;
     set_init(&w->cand, w->cd, w->ci, frontier_cap);
     set_init(&w->res, w->rd, w->ri, p->res_cap);
@@ -420,6 +503,7 @@ static void search_one(const or_space *sp, const or_params *p, or_work *w, const
         if (!visited_test_set(p, w, ep)) {
             int fa = set_push(&w->cand, dp, ep);
             int ra = set_push(&w->res, dp, ep);
+            if (p->stop && topk_insert(&w->topk, dp)) w->last_insert_hop = 0;
             tr[TR_SEED]++;
             tr[TR_VISITED]++;
             tr[TR_DEVALS]++;
@@ -430,6 +514,10 @@ static void search_one(const or_space *sp, const or_params *p, or_work *w, const
     }
 
     /* main loop: kernalGraph.h:233-289 */
+    const or_stop *st = p->stop;
+    int next_eval = st ? (st->min_hops > 0 ? st->min_hops : st->interval) : 0;
+    int votes = 0;
+    float dk_prev = INFINITY;
     for (;;) {
         if (w->cand.sz == 0) break;
         float c_min;
@@ -443,6 +531,43 @@ static void search_one(const or_space *sp, const or_params *p, or_work *w, const
             tr[TR_EARLY] = 3;
             break;
         }
+        if (st) {
+            const int hop = tr[TR_EXPANDED];
+            if (hop == next_eval) next_eval += st->interval;
+            const float dk = topk_dk(&w->topk);
+            if (dk < INFINITY && hop + st->interval == next_eval && hop >= st->min_hops) {
+                const float sc = st->dist_scale, d1 = w->topk.v[0];
+                int closer = 0; /* frontier entries (the popped one removed) closer than the k-th best */
+                for (int i = 0; i < w->cand.sz; i++) closer += w->cand.d[i] < dk;
+                float f[8];
+                f[0] = (float)hop * (1.0f / 64.0f);
+                f[1] = (c_min - dk) / sc;
+                f[2] = (dk - d1) / sc;
+                f[3] = (float)(hop - w->last_insert_hop) * (1.0f / 16.0f);
+                f[4] = (float)closer / (float)p->k;
+                f[5] = (float)w->topk.inserts / (float)p->k;
+                f[6] = (float)w->cand.sz / (float)p->ef;
+                f[7] = dk_prev < INFINITY ? (dk_prev - dk) / sc : 0.f;
+                for (int i = 0; i < 8; i++) f[i] = clamp8(f[i]);
+                dk_prev = dk;
+                w->topk.inserts = 0;
+                if (st->rec) {
+                    if (st->n_rec < st->max_evals) {
+                        float *row = st->rec + (int64_t)st->n_rec * 9;
+                        for (int i = 0; i < 8; i++) row[i] = f[i];
+                        row[8] = dk;
+                    }
+                    ((or_stop *)st)->n_rec++;
+                } else {
+                    const float logit = stop_logit(st, f);
+                    votes = logit > st->threshold ? votes + 1 : 0; /* a NaN logit never votes */
+                    if (votes >= st->confirm) {
+                        tr[TR_EARLY] = 2;
+                        break;
+                    }
+                }
+            }
+        }
         for (int j = 0; j < p->block; j++) {
             tr[TR_SLOTS]++;
             int nb = p->adj[(int64_t)cn * p->block + j];
@@ -454,6 +579,7 @@ static void search_one(const or_space *sp, const or_params *p, or_work *w, const
             tr[TR_DEVALS]++;
             int fa = set_push(&w->cand, nd, nb);
             int ra = set_push(&w->res, nd, nb);
+            if (st && topk_insert(&w->topk, nd)) w->last_insert_hop = tr[TR_EXPANDED];
             tr[TR_FPUSH] += fa;
             tr[TR_RUPD] += ra;
             peak_update(tr, w);
@@ -480,6 +606,12 @@ static int64_t row_elems(const or_space *sp) { return sp->metric == OR_METRIC_HA
  *          NULL uses the reference seeds.
  * Returns 0, or -1 on bad arguments.
  */
+static int search_all(const void *data, const int32_t *adj, int64_t n, int dim, int block, int metric,
+                      const void *queries, int nq, int k, int ef, int64_t bloom_bits, int n_seeds, int G,
+                      int ref_order, int exact_visited, int unbounded, const int32_t *entries, int n_entry,
+                      int max_hops, int result_cap, const or_stop *stop, int32_t *ids_out, float *dists_out,
+                      int32_t *trace_out);
+
 int or_search_capped(const void *data, const int32_t *adj, int64_t n, int dim, int block, int metric,
                      const void *queries, int nq, int k, int ef, int64_t bloom_bits, int n_seeds, int G,
                      int ref_order, int exact_visited, int unbounded, const int32_t *entries, int n_entry,
@@ -504,6 +636,64 @@ int or_search_capped(const void *data, const int32_t *adj, int64_t n, int dim, i
                      int ref_order, int exact_visited, int unbounded, const int32_t *entries, int n_entry,
                      int max_hops, int result_cap, int32_t *ids_out, float *dists_out, int32_t *trace_out)
 {
+    return search_all(data, adj, n, dim, block, metric, queries, nq, k, ef, bloom_bits, n_seeds, G, ref_order,
+                      exact_visited, unbounded, entries, n_entry, max_hops, result_cap, NULL, ids_out, dists_out,
+                      trace_out);
+}
+
+/* or_search_capped with the learned stop network (the MI355X search's learned
+ * bit 1): stop_w is the network in the host layout (w1 [64 x 8], b1, w2
+ * [64 x 64], b2, w3), consulted as or_stop describes; k <= 64. */
+int or_search_learned(const void *data, const int32_t *adj, int64_t n, int dim, int block, int metric,
+                      const void *queries, int nq, int k, int ef, int64_t bloom_bits, int n_seeds, int G,
+                      const int32_t *entries, int n_entry, int max_hops, int result_cap, const float *stop_w,
+                      float stop_b3, float dist_scale, float threshold, int min_hops, int interval, int confirm,
+                      int32_t *ids_out, float *dists_out, int32_t *trace_out)
+{
+    if (!stop_w || k > 64 || interval < 1 || min_hops < 0 || confirm < 1 || !(dist_scale > 0.f)) return -1;
+    or_stop st = {stop_w, stop_b3, dist_scale, threshold, min_hops, interval, confirm, NULL, 0, 0};
+    return search_all(data, adj, n, dim, block, metric, queries, nq, k, ef, bloom_bits, n_seeds, G, 0, 0, 0,
+                      entries, n_entry, max_hops, result_cap, &st, ids_out, dists_out, trace_out);
+}
+
+/* The features the learned search's trace mode records (song_learn_trace):
+ * the search with the stop network never consulted for a decision, and at
+ * each scheduled consult the 8 clamped features and the k-th best distance,
+ * rec [nq x max_evals x 9]; rec_count [nq] = rows written; final_dk [nq] =
+ * the k-th best distance at the end. */
+int or_learned_trace(const void *data, const int32_t *adj, int64_t n, int dim, int block, int metric,
+                     const void *queries, int nq, int k, int ef, int64_t bloom_bits, int n_seeds, int G,
+                     const int32_t *entries, int n_entry, float dist_scale, int min_hops, int interval,
+                     int max_evals, float *rec, int32_t *rec_count, float *final_dk)
+{
+    if (k > 64 || interval < 1 || min_hops < 0 || max_evals < 1 || !(dist_scale > 0.f) || nq < 0) return -1;
+    int32_t *ids = (int32_t *)malloc((size_t)k * sizeof(int32_t));
+    float *d = (float *)malloc((size_t)k * sizeof(float));
+    int32_t tr[TR_N];
+    int rc = 0;
+    const int64_t qstride = (metric == OR_METRIC_HAMMING ? 4 : dim) * (int64_t)(metric == OR_METRIC_HAMMING ? 8 : 4);
+    for (int qi = 0; qi < nq && rc == 0; qi++) {
+        or_stop st = {NULL, 0.f, dist_scale, 0.f, min_hops, interval, 1, rec + (int64_t)qi * max_evals * 9,
+                      max_evals, 0};
+        static const float none[4801];
+        st.w = none;
+        rc = search_all(data, adj, n, dim, block, metric, (const char *)queries + qi * qstride, 1, k, ef,
+                        bloom_bits, n_seeds, G, 0, 0, 0, entries ? entries + (int64_t)qi * n_entry : NULL, n_entry,
+                        0, 0, &st, ids, d, tr);
+        rec_count[qi] = st.n_rec < max_evals ? st.n_rec : max_evals;
+        final_dk[qi] = tr[TR_RCOUNT] >= k ? d[k - 1] : INFINITY;
+    }
+    free(ids);
+    free(d);
+    return rc;
+}
+
+static int search_all(const void *data, const int32_t *adj, int64_t n, int dim, int block, int metric,
+                      const void *queries, int nq, int k, int ef, int64_t bloom_bits, int n_seeds, int G,
+                      int ref_order, int exact_visited, int unbounded, const int32_t *entries, int n_entry,
+                      int max_hops, int result_cap, const or_stop *stop, int32_t *ids_out, float *dists_out,
+                      int32_t *trace_out)
+{
     if (max_hops < 0 || result_cap < 0) return -1;
     if (n < 1 || k < 1 || ef < 1 || block < 1 || nq < 0 || G < 1 || G > 64) return -1;
     if (!exact_visited && (bloom_bits < 64 || (bloom_bits % 64) != 0)) return -1;
@@ -514,7 +704,7 @@ int or_search_capped(const void *data, const int32_t *adj, int64_t n, int dim, i
         res_cap = result_cap;
     }
     or_space sp = {metric, dim, G, ref_order, (const float *)data, (const uint64_t *)data};
-    or_params p = {adj, n, block, k, ef, bloom_bits / 64, n_seeds, exact_visited, unbounded, max_hops, res_cap};
+    or_params p = {adj, n, block, k, ef, bloom_bits / 64, n_seeds, exact_visited, unbounded, max_hops, res_cap, stop};
     int ne = entries ? n_entry : n_seeds;
     int64_t fcap = unbounded ? n + ne + 1 : ef;
     or_work w;

@@ -377,3 +377,58 @@ def test_learned_search_other_metrics(song, datasets, metric):
         assert rec(lr.ids) > rec(plain.ids) - 0.08, (rec(lr.ids), rec(plain.ids))
     finally:
         idx.close()
+
+
+STOP_VARIANTS = {
+    "one_vote": dict(stop_min_hops=8, stop_interval=2),
+    "two_votes_hop_cap_result_cap": dict(stop_min_hops=8, stop_interval=2, stop_confirm=2, stop_max_hops=60,
+                                         result_cap=16),
+    "late_first_consult": dict(stop_min_hops=21, stop_interval=5, stop_max_hops=200),
+}
+
+
+@pytest.mark.parametrize("variant", list(STOP_VARIANTS))
+@pytest.mark.parametrize("mode", [0, 1])
+def test_stop_network_that_fires_matches_oracle(song, oracle, setup, variant, mode):
+    """The learned stop as it ends searches -- features, consult schedule, network, votes --
+    against the oracle's restatement: ids, distances and all 12 counters bit for bit, at
+    thresholds where the network stops some queries and not others."""
+    base, q, adj, idx, model = setup
+    kw = STOP_VARIANTS[variant]
+    fired_some = False
+    for thr in (-1.0, 0.5, 2.0, 4.0):
+        r = idx.search(q, 10, 128, learned=2, stop_threshold=thr, mode=mode, **kw)
+        oi, od, ot = oracle.search_learned(
+            base, adj, q, 10, 128, block=32, group=8, bloom_bits=1 << 14, stop_w=model.stop_w,
+            dist_scale=model.dist_scale, threshold=thr, min_hops=kw["stop_min_hops"], interval=kw["stop_interval"],
+            confirm=kw.get("stop_confirm", 1), max_hops=kw.get("stop_max_hops", 0), result_cap=kw.get("result_cap", 0))
+        np.testing.assert_array_equal(r.ids, oi)
+        np.testing.assert_array_equal(r.dists.view(np.uint32), od.view(np.uint32))
+        np.testing.assert_array_equal(r.trace, ot)
+        stopped = (r.trace[:, 11] == 2).mean()
+        fired_some = fired_some or 0.05 < stopped < 0.95
+    assert fired_some, "no threshold stopped a part of the queries"
+
+
+def test_trace_features_match_oracle(song, oracle, setup):
+    """The features the stop network sees (trace mode records them) equal the oracle's, bit for bit."""
+    base, q, adj, idx, model = setup
+    N = song._native
+    L = learned._bind()
+    import ctypes
+    nq, maxe = q.shape[0], 96
+    dq = idx.queries_to_device(q)
+    d_tr, d_c, d_f = N.DeviceBuffer(nq * maxe * 9 * 4), N.DeviceBuffer(nq * 4), N.DeviceBuffer(nq * 4)
+    o = N.make_opts()
+    o.stop_min_hops, o.stop_interval = 5, 3
+    N.check(L.song_learn_trace(idx.handle, dq.ptr, nq, 10, 128, ctypes.byref(o), maxe, d_tr.ptr, d_c.ptr, d_f.ptr),
+            "song_learn_trace")
+    cnt = d_c.download(np.empty(nq, np.int32))
+    fin = d_f.download(np.empty(nq, np.float32))
+    tr = d_tr.download(np.empty((nq, maxe, 9), np.float32))
+    rows, ocnt, ofin = oracle.learned_trace(base, adj, q, 10, 128, block=32, group=8, bloom_bits=1 << 14,
+                                            dist_scale=model.dist_scale, min_hops=5, interval=3, max_evals=maxe)
+    np.testing.assert_array_equal(cnt, ocnt)
+    np.testing.assert_array_equal(fin.view(np.uint32), ofin.view(np.uint32))
+    for i in range(nq):
+        np.testing.assert_array_equal(tr[i, :cnt[i]].view(np.uint32), rows[i, :cnt[i]].view(np.uint32))

@@ -14,6 +14,8 @@
   lane-group order against exact integer sums and float64.
 * Brute force against numpy.
 """
+import importlib
+
 import numpy as np
 import pytest
 
@@ -322,3 +324,78 @@ def test_result_cap(oracle, song, datasets, rcap):
         with pytest.raises(ValueError):
             oracle.search(cfg["base"], adj, cfg["queries"][:2], 10, 64, block=16, bloom_bits=1 << 15, group=8,
                           **bad)
+
+
+# ---------------------------------------------------------------- the learned stop's restatement
+
+def _stop_weights(seed):
+    learned = importlib.import_module("2020_song_ex_amd.learned")
+    rng = np.random.default_rng(seed)
+    w = learned.mlp_init(8, 1, seed=seed)
+    return (w + rng.normal(0, 0.05, size=w.shape)).astype(np.float32)
+
+
+def test_oracle_stop_logits_match_a_float64_mlp(oracle):
+    """The oracle's network (fmaf chains, pairwise output sum) against numpy's float64 MLP on
+    the same FP32 weights: within FP32 rounding.  A transposed layer or a dropped bias fails."""
+    learned = importlib.import_module("2020_song_ex_amd.learned")
+    w = _stop_weights(1)
+    f = np.random.default_rng(2).uniform(-8, 8, size=(400, 8)).astype(np.float32)
+    got = oracle.stop_logits(w, f)
+    want = learned.mlp_forward_numpy(w, f, 8, 1)[:, 0]
+    np.testing.assert_allclose(got, want, rtol=1e-5, atol=1e-5 * np.abs(want).max())
+    # planted mistake: w2 read transposed
+    wt = w.copy()
+    o = 64 * 8 + 64
+    wt[o:o + 4096] = w[o:o + 4096].reshape(64, 64).T.ravel()
+    assert np.abs(oracle.stop_logits(wt, f) - want).max() > 1e-3 * np.abs(want).max()
+
+
+@pytest.fixture(scope="module")
+def small_graph(song, datasets):
+    base, q = datasets.gmm_int(4000, 60, dim=32, n_centres=12, sigma=15.0, seed=8)
+    return base, q, song.nsw_build(base, block=16, exact_order=False)
+
+
+def test_learned_stop_that_never_fires_is_the_capped_search(oracle, small_graph):
+    base, q, adj = small_graph
+    w = _stop_weights(3)
+    kw = dict(block=16, group=8, max_hops=120, result_cap=16)
+    a = oracle.search(base, adj, q, 10, 64, **kw)
+    b = oracle.search_learned(base, adj, q, 10, 64, stop_w=w, dist_scale=50.0, threshold=np.inf, min_hops=8,
+                              interval=2, **kw)
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+
+
+@pytest.mark.parametrize("confirm", [1, 3])
+def test_learned_stop_that_always_fires_stops_on_the_schedule(oracle, small_graph, confirm):
+    """A threshold below every logit stops each query at its confirm-th consult: hops
+    min_hops, min_hops + interval, ... (k distances are known by then on this graph)."""
+    base, q, adj = small_graph
+    w = _stop_weights(4)
+    _, _, tr = oracle.search_learned(base, adj, q, 10, 64, block=16, group=8, stop_w=w, dist_scale=50.0,
+                                     threshold=-np.inf, min_hops=6, interval=3, confirm=confirm)
+    assert (tr[:, 11] == 2).all()
+    assert (tr[:, 2] == 6 + 3 * (confirm - 1)).all()
+    assert (tr[:, 3] == 16 * (tr[:, 2] - 1)).all()  # the stopping pop scans no row
+
+
+def test_learned_trace_features_follow_their_definitions(oracle, small_graph):
+    """The recorded features: consulted on the schedule, clamped to [-8, 8], the hop feature
+    hop / 64, the frontier fill in [0, 1], the k-th best never worse from one consult to the
+    next and its drop (feature 7) non-negative, the spread of the top k non-negative; the
+    final k-th best is that of the plain search."""
+    base, q, adj = small_graph
+    rows, cnt, fin = oracle.learned_trace(base, adj, q, 10, 64, block=16, group=8, dist_scale=50.0, min_hops=5,
+                                          interval=3, max_evals=200)
+    plain = oracle.search(base, adj, q, 10, 64, block=16, group=8)
+    np.testing.assert_array_equal(fin, plain[1][:, 9])
+    for i in range(q.shape[0]):
+        r = rows[i, :cnt[i]]
+        hops = r[:, 0] * 64
+        assert (hops == hops.round()).all() and ((hops - 5) % 3 == 0).all() and (np.diff(hops) >= 3).all()
+        assert (np.abs(r[:, :8]) <= 8).all()
+        assert ((r[:, 6] >= 0) & (r[:, 6] <= 1)).all()
+        assert (np.diff(r[:, 8]) <= 0).all() and (r[:, 7] >= 0).all() and (r[:, 2] >= 0).all()
+        assert r[-1, 8] >= fin[i]
