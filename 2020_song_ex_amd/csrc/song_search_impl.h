@@ -195,18 +195,18 @@ __device__ __forceinline__ uint64_t bloom_row(unsigned long long* bloom, uint32_
     return acc;
 }
 
-// The same test-then-set for rows of at most 32 ids (lanes 0..31), with the
-// hashing split over both half-waves: lane j computes hashes 0 and 1 of slot
-// j, lane j + 32 hashes 2 and 3, so each lane runs two of the four 64-bit
-// hash chains.  A slot is "all hit" when both its lanes see their two bits
-// set.  Duplicates (two fresh slots, or two hashes of one slot, on one bit)
-// are replayed slot by slot exactly as in bloom_row.  Returns the mask of
-// fresh slots (bits 0..31).
+// The same test-then-set for rows of at most 32 ids, with the hashing split
+// over both half-waves: lane j computes hashes 0 and 1 of slot j, lane j + 32
+// hashes 2 and 3, so each lane runs two of the four 64-bit hash chains.  The
+// caller gives every lane its slot's id (lane j and lane j + 32 both hold slot
+// j: the row is loaded twice from the same cache lines, so no lane has to
+// pass it to the other half).  A slot is "all hit" when both its lanes see
+// their two bits set.  Duplicates (two fresh slots, or two hashes of one slot,
+// on one bit) are replayed slot by slot exactly as in bloom_row.  Returns the
+// mask of fresh slots (bits 0..31).
 __device__ __forceinline__ uint64_t bloom_row_pairs(unsigned long long* bloom, uint32_t wmask, int wshift,
-                                                    uint32_t nbits_mask, int id, bool valid, int lane) {
+                                                    uint32_t nbits_mask, int sid, bool sv, int lane) {
     const int half = lane >> 5;
-    const int sid = __shfl(id, lane & 31, 64);
-    const bool sv = __shfl((int)valid, lane & 31, 64) != 0;
     uint32_t word[2], bitn[2];
     unsigned long long pre[2];
     bool miss[2];
@@ -302,6 +302,7 @@ __device__ __forceinline__ int dpp_i(int v) {
 // of the row below) and row_bcast:31 (rows 2 and 3 take lane 31): lane 63
 // ends with the maximum of all four rows.  Rows a broadcast skips keep the
 // identity, so the max leaves them as they were.
+constexpr int DPP_WAVE_SHR1 = 0x138;   // lane i takes lane i - 1 across the whole wave
 constexpr int DPP_ROW_BCAST15 = 0x142;
 constexpr int DPP_ROW_BCAST31 = 0x143;
 __device__ __forceinline__ int wave_max_i(int v) {
@@ -498,10 +499,12 @@ struct WSet {
 // chunks c = t, t+G, ... (c < dstride/4) of the row in order, with explicit
 // fmaf; the group sum is the xor butterfly of group_sum.  distance_passes
 // handles passes p0 .. p0+NP-1 (64/G rows each) with all their loads issued
-// before the first sum.
-template <int METRIC, int G, int QCH, int NP>
+// before the first sum; `shadow` runs between the loads and the sums (work
+// that does not depend on the rows, hidden behind their memory latency).
+template <int METRIC, int G, int QCH, int NP, class Shadow>
 __device__ __forceinline__ void distance_passes(const float4* __restrict__ data, int d4, const float4 (&q)[QCH],
-                                                int F, int p0, const int* stage_id, float* stage_d, int lane) {
+                                                int F, int p0, const int* stage_id, float* stage_d, int lane,
+                                                Shadow&& shadow) {
     constexpr int NG = 64 / G;
     const int grp = lane / G, t = lane % G;
     float4 x[NP][QCH];
@@ -519,6 +522,7 @@ __device__ __forceinline__ void distance_passes(const float4* __restrict__ data,
             else x[pp][i] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
     }
+    shadow();
 #pragma unroll
     for (int pp = 0; pp < NP; pp++) {
         float acc = 0.f;
@@ -550,29 +554,41 @@ __device__ __forceinline__ void distance_passes(const float4* __restrict__ data,
 }
 
 // NP passes at a time while that many remain, then one at a time (a row's sum
-// does not depend on which passes share a step).
-template <int METRIC, int G, int QCH, int NP>
+// does not depend on which passes share a step).  `shadow` runs once, behind
+// the first step's loads.
+template <int METRIC, int G, int QCH, int NP, class Shadow>
 __device__ __forceinline__ void float_distances(const float4* __restrict__ data, int d4,
                                                 const float4 (&q)[QCH], int F,
-                                                const int* stage_id, float* stage_d, int lane) {
+                                                const int* stage_id, float* stage_d, int lane, Shadow&& shadow) {
     constexpr int NG = 64 / G;
     const int passes = (F + NG - 1) / NG;
+    auto none = []() {};
     int p0 = 0;
-    for (; p0 + NP <= passes; p0 += NP) distance_passes<METRIC, G, QCH, NP>(data, d4, q, F, p0, stage_id, stage_d, lane);
+    if (passes >= NP) {
+        distance_passes<METRIC, G, QCH, NP>(data, d4, q, F, 0, stage_id, stage_d, lane, shadow);
+        for (p0 = NP; p0 + NP <= passes; p0 += NP)
+            distance_passes<METRIC, G, QCH, NP>(data, d4, q, F, p0, stage_id, stage_d, lane, none);
+    } else {
+        distance_passes<METRIC, G, QCH, 1>(data, d4, q, F, 0, stage_id, stage_d, lane, shadow);
+        p0 = 1;
+    }
     if constexpr (NP > 1)
-        for (; p0 < passes; p0++) distance_passes<METRIC, G, QCH, 1>(data, d4, q, F, p0, stage_id, stage_d, lane);
+        for (; p0 < passes; p0++) distance_passes<METRIC, G, QCH, 1>(data, d4, q, F, p0, stage_id, stage_d, lane, none);
 }
 
 // Hamming on 256-bit codes (4 x u64): two lanes per code, 16 bytes each
 // (warp_hamming, hashKernelGraph.h:215-224).
+template <class Shadow>
 __device__ __forceinline__ void hamming_distances(const ulonglong2* __restrict__ codes, ulonglong2 q,
-                                                  int F, const int* stage_id, float* stage_d, int lane) {
+                                                  int F, const int* stage_id, float* stage_d, int lane,
+                                                  Shadow&& shadow) {
     const int grp = lane >> 1, t = lane & 1;
     for (int p = 0; p * 32 < F; p++) {
         const int r = p * 32 + grp;
         const int node = r < F ? stage_id[r] : -1;
         ulonglong2 x = make_ulonglong2(0ull, 0ull);
         if (node >= 0) x = codes[(size_t)node * 2 + t];
+        if (p == 0) shadow();
         int c = __popcll(q.x ^ x.x) + __popcll(q.y ^ x.y);
         c = c + dppi<DPP_XOR1>(c);
         if (t == 0 && r < F) stage_d[r] = (float)c;
@@ -663,7 +679,9 @@ struct TopK {
             const int r = ffs64(m);
             const float d = readlane_f(it_d, r);
             const int pos = __popcll(__ballot(lane < k && v <= d));
-            const float up = __shfl(v, lane - 1 < 0 ? 0 : lane - 1, 64);
+            // v of lane - 1 (wave_shr:1; lane 0 keeps its own, which it never uses)
+            const float up = __int_as_float(
+                __builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), DPP_WAVE_SHR1, 0xF, 0xF, false));
             if (lane < k) v = lane < pos ? v : (lane == pos ? d : up);
             dk = readlane_f(v, k - 1);
             d1 = readlane_f(v, 0);
@@ -749,19 +767,25 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
     // and its key), then the pushes of those F items to both sets in order.
     int it_k = 0, it_id = -1;
     float it_d = INFINITY;
-    auto gather_row = [&](int id, bool valid, int width) __attribute__((always_inline)) -> int {
+    // id / valid: slot j of the row in lane j, and for rows of at most 32
+    // slots also in lane j + 32 (see bloom_row_pairs).
+    // shadow: work that does not depend on the row, run behind the row loads
+    auto gather_row = [&](int id, bool valid, int width, auto&& shadow) __attribute__((always_inline)) -> int {
         const uint64_t acc = width <= 32 ? bloom_row_pairs(bloom, wmask, wshift, nbits_mask, id, valid, lane)
                                          : bloom_row(bloom, wmask, wshift, nbits_mask, id, valid);
         const int F = __popcll(acc);
-        if (F == 0) return 0;
+        if (F == 0) {
+            shadow();
+            return 0;
+        }
         const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(acc >> 32),
                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)acc, 0u));
         if ((acc >> lane) & 1ull) stage_id[rank] = id;
         wave_lds_sync();
         if constexpr (METRIC == METRIC_HAMMING) {
-            hamming_distances((const ulonglong2*)P.data, qc, F, stage_id, stage_d, lane);
+            hamming_distances((const ulonglong2*)P.data, qc, F, stage_id, stage_d, lane, shadow);
         } else {
-            float_distances<METRIC, G, QCH, 2>((const float4*)P.data, d4, qf, F, stage_id, stage_d, lane);
+            float_distances<METRIC, G, QCH, 2>((const float4*)P.data, d4, qf, F, stage_id, stage_d, lane, shadow);
         }
         wave_lds_sync();
         it_d = lane < F ? stage_d[lane] : INFINITY;
@@ -793,6 +817,9 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
     // ties the pop can land on another node, which then loads its own row.
     int nx_key = 0;
     int pf_node = -1, nb_pref = -1;
+    // the adjacency slot this lane loads: rows of at most 32 slots are loaded
+    // into both half-waves (the layout bloom_row_pairs takes)
+    const int slot = P.block <= 32 ? (lane & 31) : lane;
     auto predict = [&](bool old_ok, int old_min, int old_id, int F) __attribute__((always_inline)) {
         bool take_new = false;
         int new_min = 0, new_id = -1;
@@ -808,23 +835,24 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
         if (!take_new && !old_ok) return;  // the candidates end empty
         nx_key = take_new ? new_min : old_min;
         pf_node = take_new ? new_id : old_id;
-        nb_pref = lane < P.block ? P.adj[(size_t)pf_node * P.block + lane] : -1;
+        nb_pref = slot < P.block ? P.adj[(size_t)pf_node * P.block + slot] : -1;
     };
 
     // Seeds: kernalGraph.h:215-231 -- ids (size_t)p * n / n_seeds, or the
     // caller's (or the planning pass's) entry list.
     {
+        const int e = P.n_entry <= 32 ? (lane & 31) : lane;
         int sid = 0;
-        bool sv = lane < P.n_entry;
+        bool sv = e < P.n_entry;
         if (sv) {
             if (P.entries) {
-                sid = P.entries[(size_t)qid * P.n_entry + lane];
+                sid = P.entries[(size_t)qid * P.n_entry + e];
                 sv = sid >= 0 && sid < P.n;
             } else {
-                sid = (int)(((uint64_t)lane * (uint64_t)P.n) / (uint64_t)P.n_entry);
+                sid = (int)(((uint64_t)e * (uint64_t)P.n) / (uint64_t)P.n_entry);
             }
         }
-        const int F = gather_row(sid, sv, P.n_entry);
+        const int F = gather_row(sid, sv, P.n_entry, []() {});
         tr_seed += F;
         tr_visited += F;
         predict(false, 0, -1, F);
@@ -838,7 +866,7 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
         const int p = cand.first_eq(c_min, lane);
         const int cn = cand.ids[p];
         int nb = nb_pref;
-        if (cn != pf_node) nb = lane < P.block ? P.adj[(size_t)cn * P.block + lane] : -1;
+        if (cn != pf_node) nb = slot < P.block ? P.adj[(size_t)cn * P.block + slot] : -1;
         cand.remove_at(p, lane);  // removing the minimum leaves the maximum unless the set empties
         if (cand.sz == 0) cand.mx_ok = false;
         tr_expanded++;
@@ -899,15 +927,16 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
                 }
             }
         }
-        // the candidates' minimum without cn, for the next prediction (while
-        // the row is in flight)
+        // the candidates' minimum without cn, for the next prediction, found
+        // while the row's vectors are in flight
         const bool old_ok = cand.sz > 0;
         int old_min = 0, old_id = -1;
-        if (old_ok) {
-            old_min = cand.vmin(lane);
-            old_id = cand.ids[cand.first_eq(old_min, lane)];
-        }
-        const int F = gather_row(nb, nb >= 0, P.block);
+        const int F = gather_row(nb, nb >= 0, P.block, [&]() __attribute__((always_inline)) {
+            if (old_ok) {
+                old_min = cand.vmin(lane);
+                old_id = cand.ids[cand.first_eq(old_min, lane)];
+            }
+        });
         tr_visited += F;
         predict(old_ok, old_min, old_id, F);
         push_row(F);
