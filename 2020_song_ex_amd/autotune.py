@@ -64,9 +64,19 @@ def measure(index, dq, nq, k, params, gt, d_ids, d_tr, repeats=3):
 
 
 def candidates(grid: dict):
-    """The grid's configurations in the order they are tried (product over the keys in order)."""
+    """The grid's configurations in the order they are tried (product over the keys in order).  A
+    value that is a dict sets several knobs together (its items replace the key)."""
     names = list(grid)
-    return [dict(zip(names, values)) for values in itertools.product(*[grid[n] for n in names])]
+    out = []
+    for values in itertools.product(*[grid[n] for n in names]):
+        c = {}
+        for n, v in zip(names, values):
+            if isinstance(v, dict):
+                c.update(v)
+            else:
+                c[n] = v
+        out.append(c)
+    return out
 
 
 def pick(tried, target_recall):

@@ -132,6 +132,7 @@ struct Plan {
     int n_entry;  // seed-row length: n_seeds, or 2 * n_seeds with bit 3
     float stop_threshold;
     int stop_min_hops, stop_interval, stop_max_hops, stop_confirm, result_cap;
+    int stop_late_hops, stop_late_interval;
     song::LaunchShape shape;
 };
 
@@ -203,6 +204,11 @@ int make_plan(const song_index* ix, int32_t nq, int32_t k, int32_t ef, const son
     p->n_entry = (o->learned & 8) ? 2 * p->n_seeds : p->n_seeds;
     if (p->n_entry > 64) return fail(SONG_ERR_ARG, "with learned bit 3, n_seeds must be <= 32");
     if (p->stop_interval < 1 || p->stop_min_hops < 0) return fail(SONG_ERR_ARG, "stop_interval >= 1, stop_min_hops >= 0");
+    p->stop_late_hops = o->stop_late_hops;
+    p->stop_late_interval = o->stop_late_interval;
+    if (o->stop_late_hops < 0) return fail(SONG_ERR_ARG, "stop_late_hops must be >= 0");
+    if (o->stop_late_hops > 0 && o->stop_late_interval < 1)
+        return fail(SONG_ERR_ARG, "stop_late_interval must be >= 1 when stop_late_hops > 0");
     if (o->learned) {
         if (!ix->has_model) return fail(SONG_ERR_STATE, "learned search needs a model (song_index_set_model)");
         if (k > 64) return fail(SONG_ERR_ARG, "learned search supports k <= 64");
@@ -244,6 +250,8 @@ song::SearchArgs make_args(const song_index* ix, const Plan& p, const void* dq, 
         a.L.stop_logit = p.stop_threshold;
         a.L.min_hops = p.stop_min_hops;
         a.L.interval = p.stop_interval;
+        a.L.late_hops = p.stop_late_hops;
+        a.L.late_interval = p.stop_late_interval;
         a.L.max_hops = p.stop_max_hops;
         a.L.confirm = p.stop_confirm;
         a.L.result_cap = p.result_cap;

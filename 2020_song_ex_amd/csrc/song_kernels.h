@@ -61,6 +61,8 @@ struct LearnedArgs {
     int32_t use_stop;
     int32_t min_hops;         // first hop at which the stop MLP is consulted
     int32_t interval;         // consult every `interval` hops
+    int32_t late_hops;        // from this hop on consult every late_interval hops (0 = never)
+    int32_t late_interval;
     int32_t max_hops;         // stop after this many expansions (0 = no cap)
     int32_t confirm;          // consecutive consults above the threshold needed to stop (>= 1)
     int32_t result_cap;       // result-set capacity below ef, without the reference's early stop (0 = ef)

@@ -885,10 +885,14 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
             // Learned termination: every `interval` hops from min_hops on, the
             // stop network sees where the search stands; trace mode records
             // the features instead (labels come from the k-th best at the end).
-            // next_eval: the next consulted hop (min_hops, then every interval)
-            if (tr_expanded == next_eval) next_eval += P.L.interval;
-            if ((tracing || stopping) && tk.dk < INFINITY && tr_expanded + P.L.interval == next_eval &&
-                tr_expanded >= P.L.min_hops) {
+            // next_eval: the next consulted hop (min_hops, then every interval;
+            // from late_hops on every late_interval)
+            bool due = false;
+            if (tr_expanded == next_eval) {
+                due = true;
+                next_eval += P.L.late_hops > 0 && tr_expanded >= P.L.late_hops ? P.L.late_interval : P.L.interval;
+            }
+            if ((tracing || stopping) && tk.dk < INFINITY && due && tr_expanded >= P.L.min_hops) {
                 const float sc = P.L.dist_scale;
                 const int dkk = dist_key<SK>(tk.dk);
                 int closer = 0;  // frontier entries still closer than the k-th best

@@ -396,10 +396,13 @@ def main():
         # ends by the stop network or the hop cap, not by the reference's test against the worst
         # of ef results (see include/song.h) -- which lets a short queue search long enough
         # the hop cap bounds the launch's last queries (their hops at full load set its end)
-        grid = {"ef": [64], "learned": [2], "stop_threshold": [1.75, 2.0, 2.25, 2.5, 3.0],
-                "bloom_bits": [1 << 14, 1 << 15], "stop_confirm": [1, 2], "stop_max_hops": [350, 400, 450, 500],
-                "result_cap": [16], "stop_min_hops": [16],
-                "stop_interval": [STOP_INTERVAL], "mode": [1], "waves_per_block": [4]}
+        # (late consults: every 16 or 32 hops from hop 150 / 200 on, where only the long searches are left)
+        grid = {"ef": [64], "learned": [2], "stop_threshold": [1.75, 2.0, 2.25, 2.5],
+                "bloom_bits": [1 << 15], "stop_confirm": [1, 2], "stop_max_hops": [400, 450, 500],
+                "result_cap": [16], "stop_min_hops": [16], "stop_interval": [STOP_INTERVAL],
+                "late": [{"stop_late_hops": h, "stop_late_interval": i} for h, i in ((0, 0), (150, 16), (200, 16),
+                                                                                    (200, 32))],
+                "mode": [1], "waves_per_block": [4]}
         given = json.loads(args.learned_config) if args.learned_config else stored.get("learned")
         if given:
             grid = {kk: [v] for kk, v in given.items()}

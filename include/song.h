@@ -142,7 +142,11 @@ typedef struct song_search_opts {
                                 ef-sized result set (kernalGraph.h:239-241);
                                 the k best are kept the same way.  No
                                 reference counterpart. */
-    int32_t reserved[2];
+    int32_t stop_late_hops;  /* with learned bit 1: from this hop on the stop
+                                network is consulted every stop_late_interval
+                                hops instead (0 = never; no reference
+                                counterpart) */
+    int32_t stop_late_interval; /* >= 1 when stop_late_hops > 0 */
 } song_search_opts;
 
 enum { SONG_MODE_PLAIN = 0, SONG_MODE_PERSISTENT = 1 };

@@ -82,13 +82,13 @@ def lib():
                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, _I64, ctypes.c_int,
                                         ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P,
                                         ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int,
-                                        ctypes.c_int, ctypes.c_int, _P, _P, _P]
+                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P, _P]
         L.or_search_learned.restype = ctypes.c_int
         L.or_stop_logits.argtypes = [_P, ctypes.c_float, _P, ctypes.c_int, _P]
         L.or_learned_trace.argtypes = [_P, _P, _I64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P,
                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, _I64, ctypes.c_int,
                                        ctypes.c_int, _P, ctypes.c_int, ctypes.c_float, ctypes.c_int,
-                                       ctypes.c_int, ctypes.c_int, _P, _P, _P]
+                                       ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P, _P]
         L.or_learned_trace.restype = ctypes.c_int
         L.or_cpu_search.argtypes = [_P, _P, _I64, ctypes.c_int, ctypes.c_int, _P, ctypes.c_int,
                                     ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P]
@@ -168,8 +168,8 @@ def search(data, adj, queries, k, ef, *, block, metric="l2", bloom_bits=1 << 14,
 
 
 def search_learned(data, adj, queries, k, ef, *, block, stop_w, dist_scale, threshold, min_hops, interval,
-                   confirm=1, max_hops=0, result_cap=0, metric="l2", bloom_bits=1 << 14, n_seeds=4, group=8,
-                   entries=None):
+                   confirm=1, max_hops=0, result_cap=0, late_hops=0, late_interval=0, metric="l2",
+                   bloom_bits=1 << 14, n_seeds=4, group=8, entries=None):
     """The search with the learned stop network (the MI355X kernel's learned bit 1) re-executed on the
     CPU: stop_w is the flat network (w1 [64 x 8], b1, w2 [64 x 64], b2, w3, then b3).
     Returns (ids [nq, k] best-first, dists [nq, k], trace [nq, 12])."""
@@ -195,14 +195,14 @@ def search_learned(data, adj, queries, k, ef, *, block, stop_w, dist_scale, thre
     rc = lib().or_search_learned(_ptr(data), _ptr(adj), n, dim, block, m, _ptr(queries), nq, k, ef, bloom_bits,
                                  n_seeds, group, _ptr(entries), ne, int(max_hops), int(result_cap), _ptr(w),
                                  float(w[4800]), float(dist_scale), float(threshold), int(min_hops), int(interval),
-                                 int(confirm), _ptr(ids), _ptr(d), _ptr(tr))
+                                 int(confirm), int(late_hops), int(late_interval), _ptr(ids), _ptr(d), _ptr(tr))
     if rc != 0:
         raise ValueError("or_search_learned rejected its arguments")
     return ids, d, tr
 
 
-def learned_trace(data, adj, queries, k, ef, *, block, dist_scale, min_hops, interval, max_evals,
-                  metric="l2", bloom_bits=1 << 14, n_seeds=4, group=8):
+def learned_trace(data, adj, queries, k, ef, *, block, dist_scale, min_hops, interval, max_evals, late_hops=0,
+                  late_interval=0, metric="l2", bloom_bits=1 << 14, n_seeds=4, group=8):
     """The learned search's trace mode re-executed on the CPU: per consulted hop the 8 features
     and the k-th best distance.  Returns (rows [nq, max_evals, 9], counts [nq], final_dk [nq])."""
     m = METRIC[metric]
@@ -217,7 +217,7 @@ def learned_trace(data, adj, queries, k, ef, *, block, dist_scale, min_hops, int
     fin = np.empty(nq, np.float32)
     rc = lib().or_learned_trace(_ptr(data), _ptr(adj), data.shape[0], dim, block, m, _ptr(queries), nq, k, ef,
                                 bloom_bits, n_seeds, group, None, 0, float(dist_scale), int(min_hops), int(interval),
-                                int(max_evals), _ptr(rows), _ptr(cnt), _ptr(fin))
+                                int(late_hops), int(late_interval), int(max_evals), _ptr(rows), _ptr(cnt), _ptr(fin))
     if rc != 0:
         raise ValueError("or_learned_trace rejected its arguments")
     return rows, cnt, fin

@@ -379,11 +379,13 @@ typedef struct {
  * them (song_search_impl.h, search_query's stop test and stop_logit):
  * w: [64 x 8] w1 | [64] b1 | [64 x 64] w2 (row j = hidden unit j) | [64] b2 |
  * [64] w3, then b3; features are consulted at hops min_hops (or interval when
- * min_hops is 0), then every interval hops, once k distances are known. */
+ * min_hops is 0), then every interval hops -- every late_interval hops once a
+ * consult falls at or after late_hops -- provided k distances are known. */
 typedef struct or_stop {
     const float *w;
     float b3, dist_scale, threshold;
     int min_hops, interval, confirm;
+    int late_hops, late_interval; /* from hop late_hops on, consult every late_interval (0: never) */
     /* recording instead of stopping (the kernel's trace mode): per consult
      * the 8 features and the k-th best distance into rec [max_evals x 9] */
     float *rec;
@@ -441,7 +443,7 @@ static float clamp8(float x) { return fminf(fmaxf(x, -8.f), 8.f); }
 /* Exposed for tests: the stop network's logit for n feature rows [n x 8]. */
 void or_stop_logits(const float *stop_w, float stop_b3, const float *features, int n, float *out)
 {
-    or_stop st = {stop_w, stop_b3, 1.f, 0.f, 0, 1, 1, NULL, 0, 0};
+    or_stop st = {stop_w, stop_b3, 1.f, 0.f, 0, 1, 1, 0, 1, NULL, 0, 0};
     for (int i = 0; i < n; i++) out[i] = stop_logit(&st, features + (int64_t)i * 8);
 }
 
@@ -533,9 +535,13 @@ static void search_one(const or_space *sp, const or_params *p, or_work *w, const
         }
         if (st) {
             const int hop = tr[TR_EXPANDED];
-            if (hop == next_eval) next_eval += st->interval;
+            int due = 0;
+            if (hop == next_eval) {
+                due = 1;
+                next_eval += st->late_hops > 0 && hop >= st->late_hops ? st->late_interval : st->interval;
+            }
             const float dk = topk_dk(&w->topk);
-            if (dk < INFINITY && hop + st->interval == next_eval && hop >= st->min_hops) {
+            if (dk < INFINITY && due && hop >= st->min_hops) {
                 const float sc = st->dist_scale, d1 = w->topk.v[0];
                 int closer = 0; /* frontier entries (the popped one removed) closer than the k-th best */
                 for (int i = 0; i < w->cand.sz; i++) closer += w->cand.d[i] < dk;
@@ -648,10 +654,12 @@ int or_search_learned(const void *data, const int32_t *adj, int64_t n, int dim, 
                       const void *queries, int nq, int k, int ef, int64_t bloom_bits, int n_seeds, int G,
                       const int32_t *entries, int n_entry, int max_hops, int result_cap, const float *stop_w,
                       float stop_b3, float dist_scale, float threshold, int min_hops, int interval, int confirm,
-                      int32_t *ids_out, float *dists_out, int32_t *trace_out)
+                      int late_hops, int late_interval, int32_t *ids_out, float *dists_out, int32_t *trace_out)
 {
     if (!stop_w || k > 64 || interval < 1 || min_hops < 0 || confirm < 1 || !(dist_scale > 0.f)) return -1;
-    or_stop st = {stop_w, stop_b3, dist_scale, threshold, min_hops, interval, confirm, NULL, 0, 0};
+    if (late_hops < 0 || (late_hops > 0 && late_interval < 1)) return -1;
+    or_stop st = {stop_w, stop_b3, dist_scale, threshold, min_hops, interval, confirm, late_hops, late_interval,
+                  NULL, 0, 0};
     return search_all(data, adj, n, dim, block, metric, queries, nq, k, ef, bloom_bits, n_seeds, G, 0, 0, 0,
                       entries, n_entry, max_hops, result_cap, &st, ids_out, dists_out, trace_out);
 }
@@ -664,17 +672,19 @@ int or_search_learned(const void *data, const int32_t *adj, int64_t n, int dim, 
 int or_learned_trace(const void *data, const int32_t *adj, int64_t n, int dim, int block, int metric,
                      const void *queries, int nq, int k, int ef, int64_t bloom_bits, int n_seeds, int G,
                      const int32_t *entries, int n_entry, float dist_scale, int min_hops, int interval,
-                     int max_evals, float *rec, int32_t *rec_count, float *final_dk)
+                     int late_hops, int late_interval, int max_evals, float *rec, int32_t *rec_count,
+                     float *final_dk)
 {
     if (k > 64 || interval < 1 || min_hops < 0 || max_evals < 1 || !(dist_scale > 0.f) || nq < 0) return -1;
+    if (late_hops < 0 || (late_hops > 0 && late_interval < 1)) return -1;
     int32_t *ids = (int32_t *)malloc((size_t)k * sizeof(int32_t));
     float *d = (float *)malloc((size_t)k * sizeof(float));
     int32_t tr[TR_N];
     int rc = 0;
     const int64_t qstride = (metric == OR_METRIC_HAMMING ? 4 : dim) * (int64_t)(metric == OR_METRIC_HAMMING ? 8 : 4);
     for (int qi = 0; qi < nq && rc == 0; qi++) {
-        or_stop st = {NULL, 0.f, dist_scale, 0.f, min_hops, interval, 1, rec + (int64_t)qi * max_evals * 9,
-                      max_evals, 0};
+        or_stop st = {NULL, 0.f, dist_scale, 0.f, min_hops, interval, 1, late_hops, late_interval,
+                      rec + (int64_t)qi * max_evals * 9, max_evals, 0};
         static const float none[4801];
         st.w = none;
         rc = search_all(data, adj, n, dim, block, metric, (const char *)queries + qi * qstride, 1, k, ef,

@@ -83,6 +83,9 @@ def test_autotuner_pick_is_deterministic():
     assert at.pick(tried, 0.96) is tried[3]
     assert at.pick(tried, 0.99) is None
     assert at.pick(list(reversed(tried)), 0.90) is tried[2]  # order decides ties
+    # a dict value sets several knobs together
+    assert at.candidates({"ef": [64], "late": [{"a": 1, "b": 2}, {"a": 0, "b": 0}]}) == [
+        {"ef": 64, "a": 1, "b": 2}, {"ef": 64, "a": 0, "b": 0}]
 
 
 def test_autotuner_finalists():

@@ -8,9 +8,13 @@ on size, checked where the timed numbers come from.
 * The persistent kernel against the plain one on all 10,000 queries.
 * The learned search's result cap with a stop that never fires, against the
   oracle's restatement of that option.
+* The tuned learned search bench.py times (profiles/tuned_configs.json), stop
+  network included, against the oracle's restatement on a sample.
 * Recall@10 of both tuned methods against the GPU brute force: >= 0.90.
 """
 import importlib
+import json
+import os
 
 import numpy as np
 import pytest
@@ -18,6 +22,8 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 learned = importlib.import_module("2020_song_ex_amd.learned")
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+TUNED = json.load(open(os.path.join(ROOT, "profiles", "tuned_configs.json")))["1000000x128 nq=10000 k=10 recall>=0.9"]
 
 
 @pytest.fixture(scope="module")
@@ -59,14 +65,26 @@ def test_persistent_equals_plain(full):
 def test_learned_search(full, oracle):
     base, q, adj, idx, gt, extra = full
     # the model bench.py trains (same queries, same settings; training is deterministic)
-    learned.fit(idx, base, extra, k=10, ef=384, stop_min_hops=16, stop_interval=8, max_evals=192,
-                entry_epochs=20, stop_epochs=4, cost_threshold=3.0)
-    # the tuned configuration reaches the benchmark's recall
-    tuned = dict(learned=2, stop_threshold=2.0, stop_confirm=1, stop_max_hops=450, result_cap=16, bloom_bits=1 << 15,
-                 stop_min_hops=16, stop_interval=8, mode=1, waves_per_block=4)
-    r = idx.search(q, 10, 64, **tuned)
+    model = learned.fit(idx, base, extra, k=10, ef=384, stop_min_hops=16, stop_interval=8, max_evals=192,
+                        entry_epochs=20, stop_epochs=4, cost_threshold=3.0)
+    # the tuned configuration bench.py times reaches the benchmark's recall
+    tuned = dict(TUNED["learned"])
+    ef = tuned.pop("ef")
+    r = idx.search(q, 10, ef, **tuned)
     assert recall(r.ids, gt) >= 0.90
     assert (r.trace[:, 2] < 436).mean() > 0.5  # fewer expansions than the plain search's mean
+    assert (r.trace[:, 11] == 2).mean() > 0.5  # most searches end by the stop network
+    # ... and equals the oracle's restatement on a sample, stop network included
+    sample = q[::100]
+    oi, od, ot = oracle.search_learned(
+        base, adj, sample, 10, ef, block=32, group=16, bloom_bits=tuned["bloom_bits"], stop_w=model.stop_w,
+        dist_scale=model.dist_scale, threshold=tuned["stop_threshold"], min_hops=tuned["stop_min_hops"],
+        interval=tuned["stop_interval"], confirm=tuned["stop_confirm"], max_hops=tuned["stop_max_hops"],
+        result_cap=tuned["result_cap"], late_hops=tuned.get("stop_late_hops", 0),
+        late_interval=tuned.get("stop_late_interval", 0))
+    np.testing.assert_array_equal(r.ids[::100], oi)
+    np.testing.assert_array_equal(r.dists[::100].view(np.uint32), od.view(np.uint32))
+    np.testing.assert_array_equal(r.trace[::100], ot)
     # result cap with a stop that never fires: the oracle's capped search with that result set
     sample = q[::100]
     s = idx.search(sample, 10, 64, learned=2, stop_threshold=1e30, stop_min_hops=16, stop_interval=8,

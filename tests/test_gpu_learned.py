@@ -384,6 +384,8 @@ STOP_VARIANTS = {
     "two_votes_hop_cap_result_cap": dict(stop_min_hops=8, stop_interval=2, stop_confirm=2, stop_max_hops=60,
                                          result_cap=16),
     "late_first_consult": dict(stop_min_hops=21, stop_interval=5, stop_max_hops=200),
+    "late_interval": dict(stop_min_hops=8, stop_interval=2, stop_late_hops=30, stop_late_interval=9,
+                          stop_max_hops=200, result_cap=16),
 }
 
 
@@ -401,7 +403,8 @@ def test_stop_network_that_fires_matches_oracle(song, oracle, setup, variant, mo
         oi, od, ot = oracle.search_learned(
             base, adj, q, 10, 128, block=32, group=8, bloom_bits=1 << 14, stop_w=model.stop_w,
             dist_scale=model.dist_scale, threshold=thr, min_hops=kw["stop_min_hops"], interval=kw["stop_interval"],
-            confirm=kw.get("stop_confirm", 1), max_hops=kw.get("stop_max_hops", 0), result_cap=kw.get("result_cap", 0))
+            confirm=kw.get("stop_confirm", 1), max_hops=kw.get("stop_max_hops", 0), result_cap=kw.get("result_cap", 0),
+            late_hops=kw.get("stop_late_hops", 0), late_interval=kw.get("stop_late_interval", 0))
         np.testing.assert_array_equal(r.ids, oi)
         np.testing.assert_array_equal(r.dists.view(np.uint32), od.view(np.uint32))
         np.testing.assert_array_equal(r.trace, ot)
@@ -410,8 +413,10 @@ def test_stop_network_that_fires_matches_oracle(song, oracle, setup, variant, mo
     assert fired_some, "no threshold stopped a part of the queries"
 
 
-def test_trace_features_match_oracle(song, oracle, setup):
-    """The features the stop network sees (trace mode records them) equal the oracle's, bit for bit."""
+@pytest.mark.parametrize("late", [(0, 0), (23, 7)])
+def test_trace_features_match_oracle(song, oracle, setup, late):
+    """The features the stop network sees (trace mode records them) equal the oracle's, bit for bit,
+    with and without a late consult interval."""
     base, q, adj, idx, model = setup
     N = song._native
     L = learned._bind()
@@ -421,13 +426,15 @@ def test_trace_features_match_oracle(song, oracle, setup):
     d_tr, d_c, d_f = N.DeviceBuffer(nq * maxe * 9 * 4), N.DeviceBuffer(nq * 4), N.DeviceBuffer(nq * 4)
     o = N.make_opts()
     o.stop_min_hops, o.stop_interval = 5, 3
+    o.stop_late_hops, o.stop_late_interval = late
     N.check(L.song_learn_trace(idx.handle, dq.ptr, nq, 10, 128, ctypes.byref(o), maxe, d_tr.ptr, d_c.ptr, d_f.ptr),
             "song_learn_trace")
     cnt = d_c.download(np.empty(nq, np.int32))
     fin = d_f.download(np.empty(nq, np.float32))
     tr = d_tr.download(np.empty((nq, maxe, 9), np.float32))
     rows, ocnt, ofin = oracle.learned_trace(base, adj, q, 10, 128, block=32, group=8, bloom_bits=1 << 14,
-                                            dist_scale=model.dist_scale, min_hops=5, interval=3, max_evals=maxe)
+                                            dist_scale=model.dist_scale, min_hops=5, interval=3, late_hops=late[0],
+                                            late_interval=late[1], max_evals=maxe)
     np.testing.assert_array_equal(cnt, ocnt)
     np.testing.assert_array_equal(fin.view(np.uint32), ofin.view(np.uint32))
     for i in range(nq):

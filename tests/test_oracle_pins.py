@@ -399,3 +399,15 @@ def test_learned_trace_features_follow_their_definitions(oracle, small_graph):
         assert ((r[:, 6] >= 0) & (r[:, 6] <= 1)).all()
         assert (np.diff(r[:, 8]) <= 0).all() and (r[:, 7] >= 0).all() and (r[:, 2] >= 0).all()
         assert r[-1, 8] >= fin[i]
+
+
+def test_learned_trace_late_schedule(oracle, small_graph):
+    """From late_hops on the consults come every late_interval hops: hops 5, 8, ..., 20, then
+    every 7 (the consult at 20 is the first at or after late_hops)."""
+    base, q, adj = small_graph
+    rows, cnt, _ = oracle.learned_trace(base, adj, q, 10, 64, block=16, group=8, dist_scale=50.0, min_hops=5,
+                                        interval=3, late_hops=19, late_interval=7, max_evals=200)
+    want = [5, 8, 11, 14, 17, 20] + list(range(27, 2000, 7))
+    for i in range(q.shape[0]):
+        hops = (rows[i, :cnt[i], 0] * 64).round().astype(int).tolist()
+        assert hops == want[:len(hops)], hops
