@@ -27,7 +27,6 @@
 #include <random>
 #include <string>
 #include <thread>
-#include <unordered_map>
 #include <vector>
 
 #include "../../include/song.h"
@@ -84,8 +83,25 @@ struct song_server {
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<song::Request*> pending;
-    std::unordered_map<int64_t, song::Request*> all;
-    std::atomic<int64_t> next_ticket{1};
+    // Outstanding requests by ticket: tickets are handed out in order under
+    // the lock, so request t sits at by_ticket[t - first_ticket] (null once
+    // taken); taken entries at the front are dropped.  No rehashing: a hash
+    // map that grows to millions of outstanding tickets stalls every
+    // submitter and the dispatcher while it rehashes under the lock.
+    std::deque<song::Request*> by_ticket;
+    int64_t first_ticket = 1;
+    int64_t next_ticket = 1;
+    song::Request* find(int64_t t) const {
+        if (t < first_ticket || t >= first_ticket + (int64_t)by_ticket.size()) return nullptr;
+        return by_ticket[(size_t)(t - first_ticket)];
+    }
+    void forget(int64_t t) {
+        by_ticket[(size_t)(t - first_ticket)] = nullptr;
+        while (!by_ticket.empty() && by_ticket.front() == nullptr) {
+            by_ticket.pop_front();
+            first_ticket++;
+        }
+    }
     int64_t inflight_bytes = 0;
     song_server_stats stats{};
     // plain fallback: with a learned search and k_max above what it serves,
@@ -545,23 +561,24 @@ static int submit_at(song_server* sv, const void* query, int32_t k, int32_t ef, 
             if (!std::isfinite(qf[i])) return sfail(SONG_ERR_ARG, "query must be finite (NaN or Inf found)");
     }
     song::Request* r = new song::Request();
-    r->ticket = sv->next_ticket.fetch_add(1);
     r->k = k;
     r->ef = std::max(ef, k);
     r->t_arrival = arrival;
     r->query.assign((const char*)query, (const char*)query + sv->row_bytes);
+    int64_t t;
     {
         std::lock_guard<std::mutex> lock(sv->mu);
         if (sv->stop) {
             delete r;
             return sfail(SONG_ERR_STATE, "server is shutting down");
         }
+        t = r->ticket = sv->next_ticket++;
         sv->pending.push_back(r);
-        sv->all[r->ticket] = r;
+        sv->by_ticket.push_back(r);
         sv->stats.submitted++;
     }
     sv->cv_work.notify_one();
-    *ticket = r->ticket;
+    *ticket = t;
     return SONG_OK;
 }
 
@@ -573,7 +590,7 @@ static int take_result(song_server* sv, song::Request* r, int32_t* ids_out, floa
     if (ids_out) std::memcpy(ids_out, r->ids.data(), r->ids.size() * 4);
     if (dists_out) std::memcpy(dists_out, r->dists.data(), r->dists.size() * 4);
     if (latency_ms) *latency_ms = ms_between(r->t_arrival, r->t_done);
-    sv->all.erase(r->ticket);
+    sv->forget(r->ticket);
     delete r;
     return 1;
 }
@@ -581,20 +598,19 @@ static int take_result(song_server* sv, song::Request* r, int32_t* ids_out, floa
 int song_server_poll(song_server* sv, int64_t ticket, int32_t* ids_out, float* dists_out, double* latency_ms) {
     if (!sv) return sfail(SONG_ERR_ARG, "null server");
     std::lock_guard<std::mutex> lock(sv->mu);
-    auto it = sv->all.find(ticket);
-    if (it == sv->all.end()) return sfail(SONG_ERR_ARG, "unknown ticket");
+    song::Request* r = sv->find(ticket);
+    if (!r) return sfail(SONG_ERR_ARG, "unknown ticket");
     if (!sv->error.empty()) return sfail(SONG_ERR_HIP, sv->error);
-    if (!it->second->done) return 0;
-    return take_result(sv, it->second, ids_out, dists_out, latency_ms);
+    if (!r->done) return 0;
+    return take_result(sv, r, ids_out, dists_out, latency_ms);
 }
 
 int song_server_wait(song_server* sv, int64_t ticket, int32_t timeout_ms, int32_t* ids_out, float* dists_out,
                      double* latency_ms) {
     if (!sv) return sfail(SONG_ERR_ARG, "null server");
     std::unique_lock<std::mutex> lock(sv->mu);
-    auto it = sv->all.find(ticket);
-    if (it == sv->all.end()) return sfail(SONG_ERR_ARG, "unknown ticket");
-    song::Request* r = it->second;
+    song::Request* r = sv->find(ticket);
+    if (!r) return sfail(SONG_ERR_ARG, "unknown ticket");
     const bool ok = sv->cv_done.wait_for(lock, std::chrono::milliseconds(timeout_ms < 0 ? 0 : timeout_ms),
                                          [&] { return r->done || !sv->error.empty(); });
     if (!sv->error.empty()) return sfail(SONG_ERR_HIP, sv->error);
@@ -623,9 +639,7 @@ void song_server_destroy(song_server* sv) {
     if (sv->rq_stop) __atomic_store_n(sv->rq_stop, 1, __ATOMIC_RELEASE);
     for (auto& s : sv->slots) free_slot(s);
     if (sv->resident) free_resident(sv);
-    for (auto& kv : sv->all) delete kv.second;
-    for (auto* r : sv->pending)
-        if (!sv->all.count(r->ticket)) delete r;
+    for (song::Request* r : sv->by_ticket) delete r;  // pending and unretired requests included
     delete sv;
 }
 

@@ -93,6 +93,27 @@ RESIDENT_OFF = ("resident serving kernel switched off: its device run returned s
                 "some ring slots and some waves did not leave at shutdown; the server refuses the mode")
 
 
+def test_tickets_taken_in_any_order(song, idx):
+    """Tickets are taken back in any order (here last first, then a gap in the middle); a taken
+    or never issued ticket is an error; results equal the uniform search."""
+    index, q = idx
+    want = index.search(q, 10, 64)
+    sv = serving.Server(index, max_batch=32, n_streams=4, k_max=10, ef_max=64)
+    try:
+        tickets = [sv.submit(q[i], 10, 64) for i in range(q.shape[0])]
+        order = list(range(len(tickets)))[::-1]
+        order = order[::2] + order[1::2]
+        for i in order:
+            got = sv.wait(tickets[i], 10)
+            assert got is not None
+            np.testing.assert_array_equal(got[0], want.ids[i])
+        for t in (tickets[0], tickets[-1], tickets[-1] + 1000, 0):
+            with pytest.raises(song.SongError, match="unknown ticket"):
+                sv.poll(t, 10)
+    finally:
+        sv.close()
+
+
 def test_resident_mode_is_refused(song, idx):
     index, q = idx
     with pytest.raises(song.SongError, match="disabled"):
