@@ -144,6 +144,9 @@ def calibration(N, args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"],
+                    help="c2: the 1M x 128 headline (default); c3: 1M x 960 plain vs autotuned persistent; "
+                         "c4: 1M 256-bit Hamming codes, plain vs learned; c5: 1.2M x 200 inner-product serving")
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1_000_000)
@@ -196,6 +199,10 @@ def main():
         return
     share = host_cpu_share()
     threads = args.threads or share["threads"]
+    if args.workload != "c2":
+        print(json.dumps({"c3": workload_c3, "c4": workload_c4, "c5": workload_c5}[args.workload](args, threads, share)),
+              flush=True)
+        return
 
     t0 = time.time()
     n_extra = 0 if (args.no_learned and args.ef) else args.train + args.valid
@@ -212,7 +219,10 @@ def main():
     if adj is None:
         t0 = time.time()
         if args.exact_graph:
-            adj = song.nsw_build(base, block=block, exact_order=True, verbose=True)
+            # the reference's sequential insertion; on integer data whose squared distances stay below
+            # 2^24 every summation order is exact, so 8 partial sums give the reference's graph bit for bit
+            exact_sums = not (np.array_equal(base, np.round(base)) and float(np.abs(base).max()) ** 2 * dim < 2 ** 24)
+            adj = song.nsw_build(base, block=block, exact_order=exact_sums, verbose=True)
         else:
             adj = song.nsw_build(base, block=block, n_threads=threads, batch_div=64, seq_prefix=1024,
                                  exact_order=False, verbose=True)
@@ -629,6 +639,253 @@ def config1_leg(song, datasets, args, threads):
            "note": "100 queries fill under 1 % of the GPU: its time here is one launch's latency"}
     log(f"config1 {out}")
     return out
+
+
+# ---------------------------------------------------------------- the other workloads
+
+def _slots(q):
+    return 1 if q <= 64 else 2 if q <= 128 else 3 if q <= 192 else 4 if q <= 256 else 6 if q <= 384 else 7 if q <= 448 else 8
+
+
+def _timed_device(idx, dq, nq, k, ef, d_ids, args, **kw):
+    """Median HIP-event kernel ms of args.steps launches after args.warmup, and the wall ms per step."""
+    for _ in range(args.warmup):
+        idx.search_device(dq, nq, k, ef, d_ids, **kw)
+    t0 = time.perf_counter()
+    ks = [idx.search_device(dq, nq, k, ef, d_ids, **kw)["kernel_ms"] for _ in range(args.steps)]
+    wall = (time.perf_counter() - t0) * 1e3 / args.steps
+    return float(np.median(ks)), wall
+
+
+def _host_qps(idx, queries, k, ef, reps=3, **kw):
+    idx.search(queries, k, ef, dists=False, trace=False, **kw)
+    return float(np.mean([queries.shape[0] / (idx.search(queries, k, ef, dists=False, trace=False, **kw)
+                                              .timing["total_ms"] / 1e3) for _ in range(reps)]))
+
+
+def _roofline(trace, row_bytes, block, k, kernel_ms, name):
+    devals = trace[:, 5].astype(np.int64).sum()
+    expanded = trace[:, 2].astype(np.int64).sum()
+    abytes = int(devals * row_bytes + expanded * 4 * block + trace.shape[0] * (row_bytes + 4 * k))
+    achieved = abytes / (kernel_ms / 1e3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": name,
+            "kernel_avg_ms": round(kernel_ms, 4), "algorithmic_bytes_per_launch": abytes}
+
+
+def _line(metric, value, unit, ms_step, args, dtype, workload, roofline, cpu, **extra):
+    return {"metric": metric, "value": round(value, 1), "unit": unit, "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": {"workload": workload, **extra},
+            "roofline": roofline, "cpu_baseline": cpu}
+
+
+def workload_c3(args, threads, share):
+    """1,000,000 x 960 values in [0, 1] around 500 Beta(2,5)-perturbed centres, 1,000 queries,
+    out-degree 32, L2, k 10: the plain kernel swept over queue sizes, and the persistent kernel
+    with waves per CU, queue size and visited-filter size picked by the autotuner on 1,000
+    held-out queries at the recall the plain kernel reaches at queue 256."""
+    song = importlib.import_module("2020_song_ex_amd")
+    datasets = importlib.import_module("2020_song_ex_amd.datasets")
+    autotune = importlib.import_module("2020_song_ex_amd.autotune")
+    N = song._native
+    t0 = time.time()
+    c = datasets.config3(nq=2000)
+    base, queries, valid = c["base"], c["queries"][:1000], c["queries"][1000:]
+    log(f"c3 data {base.shape} in {time.time() - t0:.0f}s")
+    t0 = time.time()
+    adj = song.nsw_build(base, block=32, n_threads=threads, batch_div=64, seq_prefix=1024, exact_order=False,
+                         verbose=True)
+    build_s = time.time() - t0
+    idx = song.Index(base, adj)
+    nq, k = queries.shape[0], 10
+    gt, _, _ = idx.bruteforce(queries, k)
+    vgt, _, _ = idx.bruteforce(valid, k)
+    dq = idx.queries_to_device(queries)
+    d_ids, d_tr = N.DeviceBuffer(nq * k * 4), N.DeviceBuffer(nq * 48)
+    sweep = {}
+    for ef in (64, 128, 256, 512):
+        idx.search_device(dq, nq, k, ef, d_ids, None, d_tr)
+        ids = d_ids.download(np.empty((nq, k), np.int32))
+        tr = d_tr.download(np.empty((nq, 12), np.int32))
+        kms, _ = _timed_device(idx, dq, nq, k, ef, d_ids, args)
+        sweep[ef] = {"recall_at_10": round(recall_at(ids, gt, k), 4), "kernel_ms": round(kms, 4),
+                     "qps_kernel": round(nq / (kms / 1e3), 1), "expanded_mean": float(tr[:, 2].mean())}
+        log(f"c3 plain queue {ef}: {sweep[ef]}")
+    # the plain kernel's validation recall at queue 256 is the target of the persistent tuner
+    r = idx.search(valid, k, 256, dists=False, trace=False)
+    target = recall_at(r.ids, vgt, k) - 0.002
+    grid = {"ef": [192, 256], "mode": [1], "waves_per_cu": [8, 16, 24, 32], "bloom_bits": [1 << 13, 1 << 14, 1 << 15],
+            "waves_per_block": [1, 4]}
+    res = autotune.tune(idx, valid, vgt, k, grid, target, log=log)
+    best = dict(res.best.params) if res.best else {"ef": 256, "mode": 1}
+    bef = best.pop("ef")
+    idx.search_device(dq, nq, k, bef, d_ids, None, d_tr, **best)
+    ids = d_ids.download(np.empty((nq, k), np.int32))
+    tr = d_tr.download(np.empty((nq, 12), np.int32))
+    kms, wall = _timed_device(idx, dq, nq, k, bef, d_ids, args, **best)
+    pk, _ = _timed_device(idx, dq, nq, k, bef, d_ids, args)
+    qps = nq / (wall / 1e3)
+    host_qps = _host_qps(idx, queries, k, bef, **best)
+    cpu_ids, _, _, _, cms = song.cpu_search(base, adj, queries, k, bef, n_threads=threads)
+    _, _, _, _, c1ms = song.cpu_search(base, adj, queries[:100], k, bef, n_threads=1)
+    cpu = {"value": round(100 / (c1ms / 1e3), 1), "unit": "queries/s", "cores": 1, "kind": "port",
+           "sample": f"100 queries, queue {bef}, 1 thread (song_cpu_search: Graph::search)",
+           "all_cores": {"value": round(nq / (cms / 1e3), 1), "cores": threads},
+           "recall_at_10": round(recall_at(cpu_ids, gt, k), 4), "cpu_model": cpu_model(), "host": share}
+    idx.close()
+    return _line("QPS at equal recall, 1M x 960 L2: autotuned persistent vs plain", qps, "queries/s", wall, args,
+                 "f32", f"1000000x960 Beta(2,5)-perturbed 500-centre mixture, {nq} queries, NSW out-degree 32 "
+                 f"(batched build, {build_s:.0f} s), L2, k 10",
+                 _roofline(tr, 4 * 960, 32, k, kms, f"persistent_kernel<0,{_slots(bef)},64,4,0>"), cpu,
+                 persistent={"params": {"ef": bef, **best}, "recall_at_10": round(recall_at(ids, gt, k), 4),
+                             "kernel_ms": round(kms, 4), "qps_kernel": round(nq / (kms / 1e3), 1),
+                             "qps_host_buffers": round(host_qps, 1)},
+                 plain_same_queue={"ef": bef, "kernel_ms": round(pk, 4), "qps_kernel": round(nq / (pk / 1e3), 1)},
+                 speedup_vs_plain=round(pk / kms, 3), tuner_target_recall=round(target, 4), plain_sweep=sweep,
+                 tuned=res.table())
+
+
+def _tie_recall(d, gd, k):
+    """Hits counting distance ties with the k-th true neighbour (codes tie often): per query the
+    number of returned distances <= the k-th exact distance, at most k."""
+    return float(np.mean(np.minimum((d[:, :k] <= gd[:, k - 1:k]).sum(1), k)) / k)
+
+
+def workload_c4(args, threads, share):
+    """1,000,000 unit vectors (dim 256, 2,000-centre vMF mixture, kappa 50) hashed to 256-bit codes
+    by seeded Gaussian hyperplanes, 10,000 query codes, NSW out-degree 32 on the codes, Hamming,
+    k 10: the plain kernel over queue sizes, and the learned entry + stop search trained on the
+    traces of 100,000 held-out codes and tuned on 10,000 more at the plain kernel's recall at
+    queue 256."""
+    song = importlib.import_module("2020_song_ex_amd")
+    datasets = importlib.import_module("2020_song_ex_amd.datasets")
+    autotune = importlib.import_module("2020_song_ex_amd.autotune")
+    learned = importlib.import_module("2020_song_ex_amd.learned")
+    N = song._native
+    t0 = time.time()
+    c = datasets.config4(n_extra=110_000)
+    base, queries = c["base"], c["queries"]
+    train, valid = c["extra"][:100_000], c["extra"][100_000:]
+    log(f"c4 codes {base.shape} in {time.time() - t0:.0f}s")
+    t0 = time.time()
+    adj = song.nsw_build(base, metric="hamming", block=32, n_threads=threads, batch_div=64, seq_prefix=1024,
+                         verbose=True)
+    build_s = time.time() - t0
+    idx = song.Index(base, adj, metric="hamming")
+    nq, k = queries.shape[0], 10
+    gt, gd, _ = idx.bruteforce(queries, k)
+    vgt, _, _ = idx.bruteforce(valid, k)
+    dq = idx.queries_to_device(queries)
+    d_ids, d_tr = N.DeviceBuffer(nq * k * 4), N.DeviceBuffer(nq * 48)
+    sweep = {}
+    for ef in (64, 128, 256, 512):
+        r = idx.search(queries, k, ef)
+        kms, _ = _timed_device(idx, dq, nq, k, ef, d_ids, args)
+        sweep[ef] = {"recall_at_10": round(recall_at(r.ids, gt, k), 4),
+                     "tie_recall_at_10": round(_tie_recall(r.dists, gd, k), 4), "kernel_ms": round(kms, 4),
+                     "qps_kernel": round(nq / (kms / 1e3), 1), "expanded_mean": float(r.trace[:, 2].mean())}
+        log(f"c4 plain queue {ef}: {sweep[ef]}")
+    t0 = time.time()
+    model = learned.fit(idx, base, train, k=k, ef=256, stop_min_hops=16, stop_interval=8, max_evals=192,
+                        entry_epochs=10, stop_epochs=4, cost_threshold=3.0, log=log)
+    fit_s = time.time() - t0
+    r = idx.search(valid, k, 256, dists=False, trace=False)
+    target = recall_at(r.ids, vgt, k) - 0.002
+    grid = {"ef": [128, 256], "learned": [2, 3], "stop_threshold": [1.5, 2.0, 2.5, 3.0],
+            "stop_max_hops": [600, 900], "result_cap": [16], "stop_min_hops": [16], "stop_interval": [8],
+            "mode": [1], "waves_per_block": [4]}
+    res = autotune.tune(idx, valid, vgt, k, grid, target, log=log)
+    best = dict(res.best.params) if res.best else {"ef": 256}
+    bef = best.pop("ef")
+    lr = idx.search(queries, k, bef, **best)
+    lkms, lwall = _timed_device(idx, dq, nq, k, bef, d_ids, args, **best)
+    pkms, pwall = _timed_device(idx, dq, nq, k, 256, d_ids, args)
+    host_qps = _host_qps(idx, queries, k, bef, **best)
+    idx.close()
+    return _line("QPS at equal recall, 1M 256-bit Hamming codes: learned entry + stop (persistent) vs plain",
+                 nq / (lwall / 1e3), "queries/s", lwall, args, "u64 popcount",
+                 f"1000000 256-bit codes of a 2000-centre vMF mixture (kappa 50, dim 256), {nq} queries, NSW "
+                 f"out-degree 32 on the codes (batched build, {build_s:.0f} s), Hamming, k 10",
+                 _roofline(lr.trace, 32, 32, k, lkms, f"persistent_kernel<2,{_slots(bef)},2,1,1>"), None,
+                 learned={"params": {"ef": bef, **best}, "recall_at_10": round(recall_at(lr.ids, gt, k), 4),
+                          "tie_recall_at_10": round(_tie_recall(lr.dists, gd, k), 4),
+                          "kernel_ms": round(lkms, 4), "qps_host_buffers": round(host_qps, 1),
+                          "learned_stops": float((lr.trace[:, 11] == 2).mean()),
+                          "expanded_mean": float(lr.trace[:, 2].mean()),
+                          "trained_on": f"{train.shape[0]} held-out codes' plain-search traces ({fit_s:.0f} s)"},
+                 plain_queue_256={"kernel_ms": round(pkms, 4), "qps": round(nq / (pwall / 1e3), 1),
+                                  **sweep[256]},
+                 speedup_vs_plain=round(pwall / lwall, 3), tuner_target_recall=round(target, 4), plain_sweep=sweep,
+                 tuned=res.table())
+
+
+def workload_c5(args, threads, share):
+    """1,200,000 unit vectors (dim 200, 5,000-centre vMF mixture, kappa 30) searched by inner
+    product, NSW out-degree 32: the continuous-batching server under Poisson streams of 50k,
+    200k and 500k requests/s (each about 1.5 s of arrivals, the first fifth dropped), requests
+    with k in {1, 10, 100} and queue budgets in {64, 256}; the learned stop serves k <= 64 and
+    requests with k = 100 fall back to the plain search."""
+    song = importlib.import_module("2020_song_ex_amd")
+    datasets = importlib.import_module("2020_song_ex_amd.datasets")
+    learned = importlib.import_module("2020_song_ex_amd.learned")
+    serving = importlib.import_module("2020_song_ex_amd.serving")
+    t0 = time.time()
+    c = datasets.config5(n_extra=15_000)
+    base, queries, train = c["base"], c["queries"], c["extra"]
+    log(f"c5 data {base.shape} in {time.time() - t0:.0f}s")
+    t0 = time.time()
+    adj = song.nsw_build(base, metric="ip", block=32, n_threads=threads, batch_div=64, seq_prefix=1024,
+                         exact_order=False, verbose=True)
+    build_s = time.time() - t0
+    idx = song.Index(base, adj, metric="ip")
+    nq = queries.shape[0]
+    gt, _, _ = idx.bruteforce(queries, 100)
+    learned.fit(idx, base, train, k=10, ef=256, stop_min_hops=16, stop_interval=8, max_evals=192,
+                entry_epochs=10, stop_epochs=4, cost_threshold=3.0, log=log)
+    rng = np.random.default_rng(5)
+    ks_all = rng.choice([1, 10, 100], size=nq).astype(np.int32)
+    efs_all = rng.choice([64, 256], size=nq).astype(np.int32)
+    # the learned search serves k <= 64, so its result set holds 64 (k = 100 falls back to plain)
+    search = dict(learned=2, stop_threshold=2.0, stop_min_hops=16, stop_interval=8, stop_max_hops=400,
+                  result_cap=64, bloom_bits=1 << 15, waves_per_block=4, mode=0)
+    curve = []
+    for rate in (50_000.0, 200_000.0, 500_000.0):
+        n_req = max(int(rate * args.serve_seconds), nq)
+        reps = -(-n_req // nq)
+        qs = np.tile(queries, (reps, 1))[:n_req]
+        ks = np.tile(ks_all, reps)[:n_req]
+        efs = np.tile(efs_all, reps)[:n_req]
+        sv = serving.Server(idx, max_batch=args.serve_batch, n_streams=args.serve_streams, k_max=100, ef_max=256,
+                            **search)
+        try:
+            ids, _, lat, wall = sv.run_poisson(qs, ks, efs, rate, seed=13)
+        finally:
+            sv.close()
+        steady = lat[n_req // 5:]
+        achieved = n_req / (wall / 1e3)
+        by_k = {}
+        for kk in (1, 10, 100):
+            m = ks_all == kk
+            hits = [len(set(a[:kk].tolist()) & set(g[:kk].tolist())) / kk for a, g in zip(ids[:nq][m], gt[m])]
+            by_k[str(kk)] = {"recall": round(float(np.mean(hits)), 4),
+                             "p99_ms": round(float(np.percentile(lat[:nq][m], 99)), 4)}
+        point = {"offered_qps": rate, "achieved_qps": round(achieved, 1), "overloaded": bool(achieved < 0.95 * rate),
+                 "requests": n_req, "p50_ms": round(float(np.percentile(steady, 50)), 4),
+                 "p99_ms": round(float(np.percentile(steady, 99)), 4), "by_k": by_k}
+        curve.append(point)
+        log(f"c5 serving {point}")
+    idx.close()
+    ok = [p for p in curve if not p["overloaded"]]
+    top = ok[-1] if ok else curve[0]
+    return _line("Served QPS and p99 latency, 1.2M x 200 inner product, mixed k {1, 10, 100} and queue {64, 256}",
+                 top["achieved_qps"], "queries/s", 1e3 / max(top["achieved_qps"], 1) * nq, args, "f32",
+                 f"1200000x200 unit vectors, 5000-centre vMF mixture (kappa 30), NSW out-degree 32 (batched build, "
+                 f"{build_s:.0f} s), inner product; Poisson arrivals", None, None,
+                 server={"max_batch": args.serve_batch, "streams": args.serve_streams,
+                         "hw_queues": int(os.environ["GPU_MAX_HW_QUEUES"]), "search": search,
+                         "fallback": "k > 64 runs the plain search"},
+                 p99_ms=top["p99_ms"], p50_ms=top["p50_ms"], curve=curve)
 
 
 if __name__ == "__main__":
