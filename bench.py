@@ -869,7 +869,8 @@ def workload_c5(args, threads, share):
             m = ks_all == kk
             hits = [len(set(a[:kk].tolist()) & set(g[:kk].tolist())) / kk for a, g in zip(ids[:nq][m], gt[m])]
             by_k[str(kk)] = {"recall": round(float(np.mean(hits)), 4),
-                             "p99_ms": round(float(np.percentile(lat[:nq][m], 99)), 4)}
+                             "p50_ms": round(float(np.percentile(steady[ks[n_req // 5:] == kk], 50)), 4),
+                             "p99_ms": round(float(np.percentile(steady[ks[n_req // 5:] == kk], 99)), 4)}
         point = {"offered_qps": rate, "achieved_qps": round(achieved, 1), "overloaded": bool(achieved < 0.95 * rate),
                  "requests": n_req, "p50_ms": round(float(np.percentile(steady, 50)), 4),
                  "p99_ms": round(float(np.percentile(steady, 99)), 4), "by_k": by_k}
