@@ -746,6 +746,20 @@ def workload_c3(args, threads, share):
                  tuned=res.table())
 
 
+def _oracle_cpu(base, adj, queries, gt, k, ef, metric, why, sample=200):
+    """CPU baseline where the reference has no CPU search for the metric: the oracle's restatement
+    of the reference's GPU search (same graph, seeds, queue size and Bloom filter), on one host core,
+    over the first `sample` queries."""
+    oracle = importlib.import_module("oracle")
+    q = queries[:sample]
+    t0 = time.perf_counter()
+    ids, _, _ = oracle.search(base, adj, q, k, ef, block=32, metric=metric)
+    s = time.perf_counter() - t0
+    return {"value": round(q.shape[0] / s, 1), "unit": "queries/s", "cores": 1, "kind": "port",
+            "sample": f"{q.shape[0]} queries, queue {ef}, 1 thread (the oracle's {metric} search; {why})",
+            "recall_at_10": round(recall_at(ids, gt[:q.shape[0]], k), 4), "cpu_model": cpu_model()}
+
+
 def _tie_recall(d, gd, k):
     """Hits counting distance ties with the k-th true neighbour (codes tie often): per query the
     number of returned distances <= the k-th exact distance, at most k."""
@@ -803,11 +817,12 @@ def workload_c4(args, threads, share):
     pkms, pwall = _timed_device(idx, dq, nq, k, 256, d_ids, args)
     host_qps = _host_qps(idx, queries, k, bef, **best)
     idx.close()
+    cpu = _oracle_cpu(base, adj, queries, gt, k, 256, "hamming", "the reference's bench_cpu has no Hamming search")
     return _line("QPS at equal recall, 1M 256-bit Hamming codes: learned entry + stop (persistent) vs plain",
                  nq / (lwall / 1e3), "queries/s", lwall, args, "u64 popcount",
                  f"1000000 256-bit codes of a 2000-centre vMF mixture (kappa 50, dim 256), {nq} queries, NSW "
                  f"out-degree 32 on the codes (batched build, {build_s:.0f} s), Hamming, k 10",
-                 _roofline(lr.trace, 32, 32, k, lkms, f"persistent_kernel<2,{_slots(bef)},2,1,1>"), None,
+                 _roofline(lr.trace, 32, 32, k, lkms, f"persistent_kernel<2,{_slots(bef)},2,1,1>"), cpu,
                  learned={"params": {"ef": bef, **best}, "recall_at_10": round(recall_at(lr.ids, gt, k), 4),
                           "tie_recall_at_10": round(_tie_recall(lr.dists, gd, k), 4),
                           "kernel_ms": round(lkms, 4), "qps_host_buffers": round(host_qps, 1),
@@ -877,12 +892,13 @@ def workload_c5(args, threads, share):
         curve.append(point)
         log(f"c5 serving {point}")
     idx.close()
+    cpu = _oracle_cpu(base, adj, queries, gt, 10, 256, "ip", "the reference's bench_cpu has no inner-product search")
     ok = [p for p in curve if not p["overloaded"]]
     top = ok[-1] if ok else curve[0]
     return _line("Served QPS and p99 latency, 1.2M x 200 inner product, mixed k {1, 10, 100} and queue {64, 256}",
                  top["achieved_qps"], "queries/s", 1e3 / max(top["achieved_qps"], 1) * nq, args, "f32",
                  f"1200000x200 unit vectors, 5000-centre vMF mixture (kappa 30), NSW out-degree 32 (batched build, "
-                 f"{build_s:.0f} s), inner product; Poisson arrivals", None, None,
+                 f"{build_s:.0f} s), inner product; Poisson arrivals", None, cpu,
                  server={"max_batch": args.serve_batch, "streams": args.serve_streams,
                          "hw_queues": int(os.environ["GPU_MAX_HW_QUEUES"]), "search": search,
                          "fallback": "k > 64 runs the plain search"},
