@@ -1070,8 +1070,10 @@ search_kernel(SearchArgs P) {
 
 // The resident serving grid (a request ring in host memory, claimed wave by
 // wave) is compiled out: in its device run some claimed requests read stale
-// ring rows and some waves did not leave at shutdown.  The host refuses the
-// mode while this is false.
+// ring rows and some waves did not leave at shutdown.  The stale rows have a
+// shown cause (no acquire between the claim and the request's reads, fixed
+// below); the waves that stayed have none, so the host refuses the mode while
+// this is false.
 constexpr bool kResidentServing = false;
 
 // Resident serving: the next published request, or -1 to leave (stop set, or
@@ -1124,6 +1126,15 @@ persistent_kernel(SearchArgs P) {
             const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)item >> 32));
             item = (int64_t)(((uint64_t)hi << 32) | lo);
             if (item < 0) break;
+            // Every lane reads the request (row, k, queue size) after the host
+            // wrote it: its acquire read of a tail that covers the request
+            // synchronizes with the host's release store of that tail.  The
+            // claim's relaxed read orders nothing; without this a wave read
+            // the slot's previous request (tests/resident_ring_model.cpp shows
+            // the race).  Tail only grows and lane 0 has read such a value, so
+            // the first read ends the loop.
+            while (__hip_atomic_load(Q.tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) <= item)
+                __builtin_amdgcn_s_sleep(1);
             const int slot = (int)(item % Q.ring);
             search_query<METRIC, S, G, QCH, LEARN>(P, slot, lane, lds, stop_lds);
             // every lane's ids / distances of the slot become visible to the host before its flag
