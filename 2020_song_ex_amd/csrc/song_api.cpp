@@ -78,6 +78,7 @@ struct song_index {
     int block = 0;
     void* d_data = nullptr;
     int32_t* d_adj = nullptr;
+    uint32_t* d_vpos = nullptr;  // visited-filter positions per adjacency slot, or null
     // grow-only workspace for host-buffer searches
     void* d_q = nullptr;
     int32_t* d_ids = nullptr;
@@ -223,6 +224,7 @@ song::SearchArgs make_args(const song_index* ix, const Plan& p, const void* dq, 
     song::SearchArgs a = {};
     a.data = ix->d_data;
     a.adj = ix->d_adj;
+    a.vpos = ix->d_vpos;
     a.queries = dq;
     a.ids_out = ids;
     a.dists_out = dists;
@@ -556,7 +558,37 @@ int song_index_create(const void* vecs, const int32_t* adj, int64_t n, int32_t d
     if (e != hipSuccess) return bail(hip_fail(e, "upload vectors"));
     e = hipMemcpy(ix->d_adj, adj, (size_t)n * block * 4, hipMemcpyHostToDevice);
     if (e != hipSuccess) return bail(hip_fail(e, "upload adjacency"));
+    if (block <= 32) {
+        const int rc = song_index_set_visit_table(ix, 1);
+        if (rc != SONG_OK) return bail(rc);
+    }
     *out = ix;
+    return SONG_OK;
+}
+
+int song_index_set_visit_table(song_index* ix, int32_t enabled) {
+    if (!ix) return fail(SONG_ERR_ARG, "null index");
+    std::lock_guard<std::mutex> lk(ix->mu);
+    HIP_TRY(hipSetDevice(ix->device), "hipSetDevice");
+    if (!enabled) {
+        if (ix->d_vpos) {
+            HIP_TRY(hipStreamSynchronize(ix->stream), "hipStreamSynchronize");
+            HIP_TRY(hipFree(ix->d_vpos), "hipFree visit table");
+            ix->d_vpos = nullptr;
+        }
+        return SONG_OK;
+    }
+    if (ix->block > 32) return fail(SONG_ERR_ARG, "the visit table serves rows of at most 32 slots");
+    if (ix->d_vpos) return SONG_OK;
+    uint32_t* t = nullptr;
+    HIP_TRY(hipMalloc(&t, (size_t)ix->n * 2 * ix->block * 4), "hipMalloc visit table");
+    hipError_t e = song::launch_visit_table(ix->d_adj, ix->n, ix->block, t, ix->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ix->stream);
+    if (e != hipSuccess) {
+        (void)hipFree(t);
+        return hip_fail(e, "visit table kernel");
+    }
+    ix->d_vpos = t;
     return SONG_OK;
 }
 
@@ -566,6 +598,7 @@ void song_index_destroy(song_index* ix) {
     if (ix->stream) (void)hipStreamSynchronize(ix->stream);
     if (ix->d_data) (void)hipFree(ix->d_data);
     if (ix->d_adj) (void)hipFree(ix->d_adj);
+    if (ix->d_vpos) (void)hipFree(ix->d_vpos);
     if (ix->d_q) (void)hipFree(ix->d_q);
     if (ix->d_ids) (void)hipFree(ix->d_ids);
     if (ix->d_dists) (void)hipFree(ix->d_dists);

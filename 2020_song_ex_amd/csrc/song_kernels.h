@@ -92,6 +92,12 @@ struct ResidentQueue {
 struct SearchArgs {
     const void* data;        // f32 [n x dstride] (L2/IP) or u64 [n x 4] (Hamming)
     const int32_t* adj;      // [n x block], -1 = empty slot
+    // Visited-filter positions of every adjacency slot's id, hashed once at
+    // index creation (launch_visit_table): [n x 2 x block], plane 0 holds
+    // hashes 0 | 1 << 16 and plane 1 hashes 2 | 3 << 16 (each the hash's low
+    // 16 bits, masked to the filter size at use).  Null: hash in the search.
+    // Read for rows of at most 32 slots.
+    const uint32_t* vpos;
     const void* queries;     // f32 [nq x dstride] or u64 [nq x 4]
     int32_t* ids_out;        // [nq x k], best first, -1 padded
     float* dists_out;        // [nq x k] or null
@@ -135,11 +141,9 @@ hipError_t launch_plan(const PlanArgs& P, int metric, int32_t* order, hipStream_
 // G: lanes per row (power of two); qch: 16-byte chunks per lane (G*qch*4 >= dstride).
 hipError_t launch_search(const SearchArgs& a, int metric, int S, int G, int qch, const LaunchShape& shape,
                          hipStream_t st);
-hipError_t launch_search_l2(const SearchArgs& a, int S, int G, int qch, const LaunchShape& shape, hipStream_t st);
-hipError_t launch_search_ip(const SearchArgs& a, int S, int G, int qch, const LaunchShape& shape, hipStream_t st);
-hipError_t launch_search_hamming(const SearchArgs& a, int S, int G, int qch, const LaunchShape& shape,
-                                 hipStream_t st);
-
+// The visited-filter position table of SearchArgs::vpos for an adjacency
+// [n x block] already in device memory (out: n * 2 * block words).
+hipError_t launch_visit_table(const int32_t* adj, int64_t n, int block, uint32_t* out, hipStream_t st);
 // BitHash codes of n vectors (uint64 [n x 4]) from R [256 x dim], dim <= 2048,
 // all in device memory: the sequential float dot products of song_bithash_project.
 hipError_t launch_bithash(const float* R, int dim, const float* X, int64_t n, uint64_t* codes, hipStream_t st);

@@ -115,20 +115,25 @@ __device__ __forceinline__ int ffs64(uint64_t m) { return __ffsll((unsigned long
 
 // ---------------------------------------------------------------- Bloom filter
 
-// BF_RAND, bloomFilter.h:14-19.
-__device__ __forceinline__ uint32_t bloom_pos(int h, uint32_t id, uint32_t nbits_mask) {
+// BF_RAND, bloomFilter.h:14-19.  Returns the low 32 bits of hash h of id;
+// the bit count is a power of two, so the position is these bits masked.
+__device__ __forceinline__ uint32_t bloom_hash(int h, uint32_t id) {
     constexpr uint64_t R[8] = {
         0x4bcb391f924ed183ULL, 0xa0ab69ccd854fc0aULL, 0x91086b9cecf5e3b7ULL, 0xc68e01641bead407ULL,
         0x3a7b976128a30449ULL, 0x6d122efabfc4d99fULL, 0xe6700ef8715030e2ULL, 0x80dd0c3bffcfb45bULL,
     };
     // bf_hash, bloomFilter.h:23-30.  id < 2^31, so the first x ^= x >> 33 is
-    // the identity; the bit count is a power of two, so "% nbits" is a mask.
+    // the identity.
     uint64_t x = (uint64_t)id;
     x *= R[2 * h];
     x ^= x >> 33;
     x *= R[2 * h + 1];
     x ^= x >> 33;
-    return (uint32_t)x & nbits_mask;
+    return (uint32_t)x;
+}
+// "% nbits" of bf_hash (bloomFilter.h:30) for a power-of-two bit count.
+__device__ __forceinline__ uint32_t bloom_pos(int h, uint32_t id, uint32_t nbits_mask) {
+    return bloom_hash(h, id) & nbits_mask;
 }
 
 // Test-then-set of one row of ids in slot order (the reference's per-slot
@@ -203,9 +208,13 @@ __device__ __forceinline__ uint64_t bloom_row(unsigned long long* bloom, uint32_
 // pass it to the other half).  A slot is "all hit" when both its lanes see
 // their two bits set.  Duplicates (two fresh slots, or two hashes of one slot,
 // on one bit) are replayed slot by slot exactly as in bloom_row.  Returns the
-// mask of fresh slots (bits 0..31).
+// mask of fresh slots (bits 0..31).  PRE: the lane's two positions come
+// precomputed in vp (hash 2 * half + t in bits 16 t .. 16 t + 15, see
+// SearchArgs::vpos) instead of being hashed here.
+template <bool PRE>
 __device__ __forceinline__ uint64_t bloom_row_pairs(unsigned long long* bloom, uint32_t wmask, int wshift,
-                                                    uint32_t nbits_mask, int sid, bool sv, int lane) {
+                                                    uint32_t nbits_mask, int sid, bool sv, int lane,
+                                                    uint32_t vp) {
     const int half = lane >> 5;
     uint32_t word[2], bitn[2];
     unsigned long long pre[2];
@@ -214,7 +223,11 @@ __device__ __forceinline__ uint64_t bloom_row_pairs(unsigned long long* bloom, u
 #pragma unroll
     for (int t = 0; t < 2; t++) {
         uint32_t pos = 0u;
-        if (sv) pos = half ? bloom_pos(2 + t, (uint32_t)sid, nbits_mask) : bloom_pos(t, (uint32_t)sid, nbits_mask);
+        if constexpr (PRE) {
+            pos = (vp >> (16 * t)) & nbits_mask;
+        } else {
+            if (sv) pos = half ? bloom_pos(2 + t, (uint32_t)sid, nbits_mask) : bloom_pos(t, (uint32_t)sid, nbits_mask);
+        }
         word[t] = pos & wmask;
         bitn[t] = pos >> wshift;
         pre[t] = sv ? bloom[word[t]] : 0ull;
@@ -770,9 +783,14 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
     // id / valid: slot j of the row in lane j, and for rows of at most 32
     // slots also in lane j + 32 (see bloom_row_pairs).
     // shadow: work that does not depend on the row, run behind the row loads
-    auto gather_row = [&](int id, bool valid, int width, auto&& shadow) __attribute__((always_inline)) -> int {
-        const uint64_t acc = width <= 32 ? bloom_row_pairs(bloom, wmask, wshift, nbits_mask, id, valid, lane)
-                                         : bloom_row(bloom, wmask, wshift, nbits_mask, id, valid);
+    // vp: the slot's precomputed filter positions when `pre` (adjacency rows
+    // of at most 32 slots with a position table); seeds are hashed here.
+    auto gather_row = [&](int id, bool valid, int width, bool pre, uint32_t vp,
+                          auto&& shadow) __attribute__((always_inline)) -> int {
+        const uint64_t acc =
+            pre ? bloom_row_pairs<true>(bloom, wmask, wshift, nbits_mask, id, valid, lane, vp)
+                : width <= 32 ? bloom_row_pairs<false>(bloom, wmask, wshift, nbits_mask, id, valid, lane, 0u)
+                              : bloom_row(bloom, wmask, wshift, nbits_mask, id, valid);
         const int F = __popcll(acc);
         if (F == 0) {
             shadow();
@@ -817,9 +835,17 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
     // ties the pop can land on another node, which then loads its own row.
     int nx_key = 0;
     int pf_node = -1, nb_pref = -1;
+    uint32_t vp_pref = 0u;
     // the adjacency slot this lane loads: rows of at most 32 slots are loaded
-    // into both half-waves (the layout bloom_row_pairs takes)
+    // into both half-waves (the layout bloom_row_pairs takes), and with the
+    // position table lane j of half h also loads plane h of slot j
     const int slot = P.block <= 32 ? (lane & 31) : lane;
+    const bool use_vpos = P.vpos != nullptr && P.block <= 32;
+    const int vslot = (lane >> 5) * P.block + slot;  // within a node's 2 x block words
+    auto load_row = [&](int node, int& nbv, uint32_t& vpv) __attribute__((always_inline)) {
+        nbv = slot < P.block ? P.adj[(size_t)node * P.block + slot] : -1;
+        if (use_vpos) vpv = slot < P.block ? P.vpos[(size_t)node * 2 * P.block + vslot] : 0u;
+    };
     auto predict = [&](bool old_ok, int old_min, int old_id, int F) __attribute__((always_inline)) {
         bool take_new = false;
         int new_min = 0, new_id = -1;
@@ -835,7 +861,7 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
         if (!take_new && !old_ok) return;  // the candidates end empty
         nx_key = take_new ? new_min : old_min;
         pf_node = take_new ? new_id : old_id;
-        nb_pref = slot < P.block ? P.adj[(size_t)pf_node * P.block + slot] : -1;
+        load_row(pf_node, nb_pref, vp_pref);
     };
 
     // Seeds: kernalGraph.h:215-231 -- ids (size_t)p * n / n_seeds, or the
@@ -852,7 +878,7 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
                 sid = (int)(((uint64_t)e * (uint64_t)P.n) / (uint64_t)P.n_entry);
             }
         }
-        const int F = gather_row(sid, sv, P.n_entry, []() {});
+        const int F = gather_row(sid, sv, P.n_entry, false, 0u, []() {});
         tr_seed += F;
         tr_visited += F;
         predict(false, 0, -1, F);
@@ -866,7 +892,8 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
         const int p = cand.first_eq(c_min, lane);
         const int cn = cand.ids[p];
         int nb = nb_pref;
-        if (cn != pf_node) nb = slot < P.block ? P.adj[(size_t)cn * P.block + slot] : -1;
+        uint32_t vp = vp_pref;
+        if (cn != pf_node) load_row(cn, nb, vp);
         cand.remove_at(p, lane);  // removing the minimum leaves the maximum unless the set empties
         if (cand.sz == 0) cand.mx_ok = false;
         tr_expanded++;
@@ -935,7 +962,7 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
         // while the row's vectors are in flight
         const bool old_ok = cand.sz > 0;
         int old_min = 0, old_id = -1;
-        const int F = gather_row(nb, nb >= 0, P.block, [&]() __attribute__((always_inline)) {
+        const int F = gather_row(nb, nb >= 0, P.block, use_vpos, vp, [&]() __attribute__((always_inline)) {
             if (old_ok) {
                 old_min = cand.vmin(lane);
                 old_id = cand.ids[cand.first_eq(old_min, lane)];
@@ -1222,20 +1249,6 @@ static hipError_t launch_g(const SearchArgs& a, int G, int qch, const LaunchShap
         }
         return hipErrorInvalidValue;
     }
-}
-
-template <int METRIC>
-hipError_t launch_s(const SearchArgs& a, int S, int G, int qch, const LaunchShape& wpb, hipStream_t st) {
-    switch (S) {
-    case 1: return launch_g<METRIC, 1>(a, G, qch, wpb, st);
-    case 2: return launch_g<METRIC, 2>(a, G, qch, wpb, st);
-    case 3: return launch_g<METRIC, 3>(a, G, qch, wpb, st);
-    case 4: return launch_g<METRIC, 4>(a, G, qch, wpb, st);
-    case 6: return launch_g<METRIC, 6>(a, G, qch, wpb, st);
-    case 7: return launch_g<METRIC, 7>(a, G, qch, wpb, st);
-    case 8: return launch_g<METRIC, 8>(a, G, qch, wpb, st);
-    }
-    return hipErrorInvalidValue;
 }
 
 }  // namespace song

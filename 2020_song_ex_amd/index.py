@@ -162,6 +162,11 @@ class Index:
     def handle(self):
         return self._h
 
+    def set_visit_table(self, enabled):
+        """Build (True) or free (False) the per-slot visited-filter positions the search reads
+        instead of hashing each neighbour id (built by default for rows of at most 32 slots)."""
+        N.check(N.lib().song_index_set_visit_table(self._h, 1 if enabled else 0), "song_index_set_visit_table")
+
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             N.lib().song_index_destroy(self._h)

@@ -216,6 +216,16 @@ int song_index_create(const void* vecs, const int32_t* adj, int64_t n, int32_t d
 /* Frees the device copy (~KernelGraph, kernalGraph.h:45). */
 void song_index_destroy(song_index* index);
 
+/* The index's visit table: for every adjacency slot, the visited-filter
+ * positions of its id (the low 16 bits of the four bf_hash values,
+ * bloomFilter.h:23-30), hashed once on the GPU, n * block * 8 bytes of HBM.
+ * The search then reads a row's positions beside its ids instead of hashing
+ * them in the hop loop; results are the same either way.  song_index_create
+ * builds it when block <= 32.  enabled = 0 frees it (searches hash again),
+ * 1 builds it (block <= 32 only).  Not while a search or server on this index
+ * runs.  No reference counterpart. */
+int song_index_set_visit_table(song_index* index, int32_t enabled);
+
 /* Floats (or uint64 words for HAMMING) per stored query row on the device:
  * dim rounded up to a multiple of 4 (4 for HAMMING). */
 int song_index_row_stride(const song_index* index, int32_t* out);

@@ -76,6 +76,26 @@ def test_bloom_sizes(song, oracle, c1, bits):
     assert_same(r, o, f"bloom={bits}")
 
 
+@pytest.mark.parametrize("bits", [512, 1 << 15])
+def test_visit_table_off(song, oracle, c1, bits):
+    """Every other test searches with the index's visit table (the filter positions hashed once at
+    index creation); without it the search hashes each neighbour id itself: the same results."""
+    cfg, adj = c1
+    idx = song.Index(cfg["base"], adj)
+    try:
+        on = idx.search(cfg["queries"], 10, 64, bloom_bits=bits)
+        idx.set_visit_table(False)
+        off = idx.search(cfg["queries"], 10, 64, bloom_bits=bits)
+        idx.set_visit_table(True)
+        again = idx.search(cfg["queries"], 10, 64, bloom_bits=bits)
+    finally:
+        idx.close()
+    o = oracle.search(cfg["base"], adj, cfg["queries"], 10, 64, block=16, bloom_bits=bits,
+                      group=lanes_for(cfg["base"].shape[1]))
+    for r, label in ((on, "table"), (off, "hashed"), (again, "table rebuilt")):
+        assert_same(r, o, f"{label}, bloom={bits}")
+
+
 @pytest.mark.parametrize("lanes,dim", [(2, 30), (4, 30), (8, 30), (16, 30), (32, 30), (64, 30),
                                        (8, 100), (16, 100), (64, 100)])
 def test_lane_groups(song, oracle, datasets, lanes, dim):
