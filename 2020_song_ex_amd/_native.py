@@ -47,7 +47,7 @@ EXPORTS = (
     "song_abi_version", "song_last_error", "song_device_count", "song_nsw_build",
     "song_graph_save", "song_graph_load_header", "song_graph_load", "song_bithash_matrix",
     "song_bithash_project", "song_bithash_project_device", "song_index_create", "song_index_destroy",
-    "song_index_row_stride", "song_index_set_visit_table",
+    "song_index_row_stride", "song_index_set_visit_table", "song_index_set_wide_offsets",
     "song_search", "song_search_device", "song_dev_alloc", "song_dev_free", "song_memcpy_h2d",
     "song_memcpy_d2h", "song_queries_to_device", "song_index_bruteforce",
     "song_search_device_mixed", "song_dev_copy", "song_index_set_model",
@@ -89,6 +89,7 @@ def lib():
     L.song_index_destroy.restype = None
     L.song_index_row_stride.argtypes = [_P, _P]
     L.song_index_set_visit_table.argtypes = [_P, _I32]
+    L.song_index_set_wide_offsets.argtypes = [_P, _I32]
     L.song_search.argtypes = [_P, _P, _I32, _I32, _I32, _P, _P, _P, _P, _P, _P]
     L.song_search_device.argtypes = [_P, _P, _I32, _I32, _I32, _P, _P, _P, _P, _P]
     L.song_cpu_search.argtypes = [_P, _P, _I64, _I32, _I32, _P, _I32, _I32, _I32, _I32, _I32, _P, _P, _P, _P,

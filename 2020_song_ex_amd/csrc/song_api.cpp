@@ -79,6 +79,7 @@ struct song_index {
     void* d_data = nullptr;
     int32_t* d_adj = nullptr;
     uint32_t* d_vpos = nullptr;  // visited-filter positions per adjacency slot, or null
+    bool wide_offsets = false;   // address rows with 64-bit offsets even where 32-bit ones do
     // grow-only workspace for host-buffer searches
     void* d_q = nullptr;
     int32_t* d_ids = nullptr;
@@ -232,6 +233,8 @@ song::SearchArgs make_args(const song_index* ix, const Plan& p, const void* dq, 
     a.entries = nullptr;
     a.n = ix->n;
     a.dstride = ix->dstride;
+    a.off32 = !ix->wide_offsets && ix->n < (int64_t(1) << 23) && ix->n * (int64_t)row_bytes(ix) < (int64_t(1) << 31)
+                  ? 1 : 0;
     a.block = ix->block;
     a.k = k;
     a.ef = p.ef;
@@ -563,6 +566,13 @@ int song_index_create(const void* vecs, const int32_t* adj, int64_t n, int32_t d
         if (rc != SONG_OK) return bail(rc);
     }
     *out = ix;
+    return SONG_OK;
+}
+
+int song_index_set_wide_offsets(song_index* ix, int32_t enabled) {
+    if (!ix) return fail(SONG_ERR_ARG, "null index");
+    std::lock_guard<std::mutex> lk(ix->mu);
+    ix->wide_offsets = enabled != 0;
     return SONG_OK;
 }
 

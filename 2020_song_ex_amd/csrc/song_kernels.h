@@ -105,6 +105,7 @@ struct SearchArgs {
     const int32_t* entries;  // [nq x n_entry] entry ids, or null for the evenly spaced seeds
     int64_t n;
     int32_t dstride;         // floats per stored row (dim rounded up to 4)
+    int32_t off32;           // 1: n < 2^23 and n * row bytes < 2^31 (32-bit row offsets)
     int32_t block;           // adjacency slots per node (<= 64)
     int32_t k, ef, nq;
     int32_t bloom_words;     // 64-bit words, power of two

@@ -514,7 +514,13 @@ struct WSet {
 // handles passes p0 .. p0+NP-1 (64/G rows each) with all their loads issued
 // before the first sum; `shadow` runs between the loads and the sums (work
 // that does not depend on the rows, hidden behind their memory latency).
-template <int METRIC, int G, int QCH, int NP, class Shadow>
+// Groups past the last row read row 0 and drop its sum, so no load waits
+// behind a branch.  FULL: the group's lanes cover the row exactly
+// (G * QCH == dstride / 4), so no chunk needs a range test.  OFF32: row
+// offsets fit 31 bits and node ids 23 bits (the host checks), so a row's
+// byte offset is one 24-bit multiply and the load takes a 32-bit offset from
+// the base address instead of a 64-bit address per lane.
+template <int METRIC, int G, int QCH, int NP, bool FULL, bool OFF32, class Shadow>
 __device__ __forceinline__ void distance_passes(const float4* __restrict__ data, int d4, const float4 (&q)[QCH],
                                                 int F, int p0, const int* stage_id, float* stage_d, int lane,
                                                 Shadow&& shadow) {
@@ -524,17 +530,32 @@ __device__ __forceinline__ void distance_passes(const float4* __restrict__ data,
     int rr[NP];
 #pragma unroll
     for (int pp = 0; pp < NP; pp++) {
-        const int r = (p0 + pp) * NG + grp;
+        const int r = (p0 + pp) * NG + grp;  // < 64: passes cover at most the 64 staged rows
         rr[pp] = r;
-        const int node = r < F ? stage_id[r] : -1;
-        const float4* row = data + (size_t)(node < 0 ? 0 : node) * d4;
+        const int sid = stage_id[r];
+        const int node = r < F ? sid : 0;
+        if constexpr (OFF32) {
+            const uint32_t off = (uint32_t)__mul24(node, d4 * 16) + (uint32_t)t * 16u;
+            const char* row = (const char*)data + off;
 #pragma unroll
-        for (int i = 0; i < QCH; i++) {
-            const int c = t + i * G;
-            if (node >= 0 && c < d4) x[pp][i] = row[c];
-            else x[pp][i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            for (int i = 0; i < QCH; i++) {
+                if (FULL || t + i * G < d4) x[pp][i] = *(const float4*)(row + i * G * 16);
+                else x[pp][i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+        } else {
+            const float4* row = data + (size_t)node * d4;
+#pragma unroll
+            for (int i = 0; i < QCH; i++) {
+                const int c = t + i * G;
+                if (FULL || c < d4) x[pp][i] = row[c];
+                else x[pp][i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
         }
     }
+    // every row load issues before any sum: the scheduler would otherwise
+    // start the first pass's sums, waiting on its loads, before issuing the
+    // next pass's loads
+    __builtin_amdgcn_sched_barrier(0);
     shadow();
 #pragma unroll
     for (int pp = 0; pp < NP; pp++) {
@@ -542,7 +563,7 @@ __device__ __forceinline__ void distance_passes(const float4* __restrict__ data,
 #pragma unroll
         for (int i = 0; i < QCH; i++) {
             const int c = t + i * G;
-            if (c < d4) {
+            if (FULL || c < d4) {
                 if constexpr (METRIC == METRIC_IP) {
                     acc = __builtin_fmaf(q[i].x, x[pp][i].x, acc);
                     acc = __builtin_fmaf(q[i].y, x[pp][i].y, acc);
@@ -569,24 +590,38 @@ __device__ __forceinline__ void distance_passes(const float4* __restrict__ data,
 // NP passes at a time while that many remain, then one at a time (a row's sum
 // does not depend on which passes share a step).  `shadow` runs once, behind
 // the first step's loads.
-template <int METRIC, int G, int QCH, int NP, class Shadow>
-__device__ __forceinline__ void float_distances(const float4* __restrict__ data, int d4,
-                                                const float4 (&q)[QCH], int F,
-                                                const int* stage_id, float* stage_d, int lane, Shadow&& shadow) {
+template <int METRIC, int G, int QCH, int NP, bool FULL, bool OFF32, class Shadow>
+__device__ __forceinline__ void float_distances_v(const float4* __restrict__ data, int d4,
+                                                  const float4 (&q)[QCH], int F,
+                                                  const int* stage_id, float* stage_d, int lane, Shadow&& shadow) {
     constexpr int NG = 64 / G;
     const int passes = (F + NG - 1) / NG;
     auto none = []() {};
     int p0 = 0;
     if (passes >= NP) {
-        distance_passes<METRIC, G, QCH, NP>(data, d4, q, F, 0, stage_id, stage_d, lane, shadow);
+        distance_passes<METRIC, G, QCH, NP, FULL, OFF32>(data, d4, q, F, 0, stage_id, stage_d, lane, shadow);
         for (p0 = NP; p0 + NP <= passes; p0 += NP)
-            distance_passes<METRIC, G, QCH, NP>(data, d4, q, F, p0, stage_id, stage_d, lane, none);
+            distance_passes<METRIC, G, QCH, NP, FULL, OFF32>(data, d4, q, F, p0, stage_id, stage_d, lane, none);
     } else {
-        distance_passes<METRIC, G, QCH, 1>(data, d4, q, F, 0, stage_id, stage_d, lane, shadow);
+        distance_passes<METRIC, G, QCH, 1, FULL, OFF32>(data, d4, q, F, 0, stage_id, stage_d, lane, shadow);
         p0 = 1;
     }
     if constexpr (NP > 1)
-        for (; p0 < passes; p0++) distance_passes<METRIC, G, QCH, 1>(data, d4, q, F, p0, stage_id, stage_d, lane, none);
+        for (; p0 < passes; p0++)
+            distance_passes<METRIC, G, QCH, 1, FULL, OFF32>(data, d4, q, F, p0, stage_id, stage_d, lane, none);
+}
+
+// off32: the index's rows are addressed with 32-bit offsets (SearchArgs::off32).
+template <int METRIC, int G, int QCH, int NP, class Shadow>
+__device__ __forceinline__ void float_distances(const float4* __restrict__ data, int d4, bool off32,
+                                                const float4 (&q)[QCH], int F,
+                                                const int* stage_id, float* stage_d, int lane, Shadow&& shadow) {
+    if (off32 && d4 == G * QCH)
+        float_distances_v<METRIC, G, QCH, NP, true, true>(data, d4, q, F, stage_id, stage_d, lane, shadow);
+    else if (off32)
+        float_distances_v<METRIC, G, QCH, NP, false, true>(data, d4, q, F, stage_id, stage_d, lane, shadow);
+    else
+        float_distances_v<METRIC, G, QCH, NP, false, false>(data, d4, q, F, stage_id, stage_d, lane, shadow);
 }
 
 // Hamming on 256-bit codes (4 x u64): two lanes per code, 16 bytes each
@@ -803,7 +838,8 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
         if constexpr (METRIC == METRIC_HAMMING) {
             hamming_distances((const ulonglong2*)P.data, qc, F, stage_id, stage_d, lane, shadow);
         } else {
-            float_distances<METRIC, G, QCH, 2>((const float4*)P.data, d4, qf, F, stage_id, stage_d, lane, shadow);
+            float_distances<METRIC, G, QCH, 2>((const float4*)P.data, d4, P.off32 != 0, qf, F, stage_id, stage_d, lane,
+                                               shadow);
         }
         wave_lds_sync();
         it_d = lane < F ? stage_d[lane] : INFINITY;

@@ -167,6 +167,10 @@ class Index:
         instead of hashing each neighbour id (built by default for rows of at most 32 slots)."""
         N.check(N.lib().song_index_set_visit_table(self._h, 1 if enabled else 0), "song_index_set_visit_table")
 
+    def set_wide_offsets(self, enabled):
+        """Address vector rows with 64-bit offsets (the path of indexes above 2 GiB) on this index too."""
+        N.check(N.lib().song_index_set_wide_offsets(self._h, 1 if enabled else 0), "song_index_set_wide_offsets")
+
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             N.lib().song_index_destroy(self._h)

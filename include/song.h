@@ -226,6 +226,12 @@ void song_index_destroy(song_index* index);
  * runs.  No reference counterpart. */
 int song_index_set_visit_table(song_index* index, int32_t enabled);
 
+/* Row addressing of the search.  By default rows are addressed with 32-bit
+ * offsets when n < 2^23 and the vectors take under 2 GiB, else with 64-bit
+ * ones; enabled = 1 takes the 64-bit path on any index (the same results).
+ * No reference counterpart. */
+int song_index_set_wide_offsets(song_index* index, int32_t enabled);
+
 /* Floats (or uint64 words for HAMMING) per stored query row on the device:
  * dim rounded up to a multiple of 4 (4 for HAMMING). */
 int song_index_row_stride(const song_index* index, int32_t* out);

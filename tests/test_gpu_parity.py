@@ -96,6 +96,22 @@ def test_visit_table_off(song, oracle, c1, bits):
         assert_same(r, o, f"{label}, bloom={bits}")
 
 
+@pytest.mark.parametrize("dim", [128, 100])
+def test_wide_offsets(song, oracle, datasets, dim):
+    """Indexes above 2 GiB address rows with 64-bit offsets; forced on a small index it gives the
+    oracle's results too, for rows the lane groups cover exactly (128) and with a ragged end (100)."""
+    base, q = datasets.gmm_int(3000, 40, dim=dim, n_centres=20, sigma=20.0, seed=3)
+    adj = song.nsw_build(base, block=16, exact_order=False)
+    idx = song.Index(base, adj)
+    try:
+        idx.set_wide_offsets(True)
+        r = idx.search(q, 10, 64, bloom_bits=1 << 15)
+    finally:
+        idx.close()
+    o = oracle.search(base, adj, q, 10, 64, block=16, bloom_bits=1 << 15, group=lanes_for(dim))
+    assert_same(r, o, f"wide offsets, dim {dim}")
+
+
 @pytest.mark.parametrize("lanes,dim", [(2, 30), (4, 30), (8, 30), (16, 30), (32, 30), (64, 30),
                                        (8, 100), (16, 100), (64, 100)])
 def test_lane_groups(song, oracle, datasets, lanes, dim):
