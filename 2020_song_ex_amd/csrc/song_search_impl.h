@@ -230,7 +230,13 @@ __device__ __forceinline__ uint64_t bloom_row_pairs(unsigned long long* bloom, u
         }
         word[t] = pos & wmask;
         bitn[t] = pos >> wshift;
-        pre[t] = sv ? bloom[word[t]] : 0ull;
+        if constexpr (PRE) {
+            // the word index is in range for any lane: read without a branch
+            const unsigned long long w = bloom[word[t]];
+            pre[t] = sv ? w : 0ull;
+        } else {
+            pre[t] = sv ? bloom[word[t]] : 0ull;
+        }
         miss[t] = !((pre[t] >> bitn[t]) & 1ull);
         mine = mine && !miss[t];
     }
@@ -441,10 +447,11 @@ struct WSet {
     }
     // Remove entry p, filling the hole with the last entry (gpuHeap.h:44-48);
     // the vacated last slot becomes empty.  The caller keeps mx_ok as fits.
-    __device__ __forceinline__ void remove_at(int p, int lane) {
+    __device__ __forceinline__ void remove_at(int p, int lane) { remove_at(p, ids[sz - 1], lane); }
+    // The same with the last entry's id already read (li = ids[sz - 1]).
+    __device__ __forceinline__ void remove_at(int p, int li, int lane) {
         const int last = sz - 1;
         const int lk = get_d(last);
-        const int li = ids[last];
         put(p, lk, li, lane);
         put_d(last, KEY_EMPTY, lane);
         sz--;
@@ -467,11 +474,9 @@ struct WSet {
             }
         }
     }
-    // Push items it_k/it_id held by lanes 0..F-1, in lane order, with the
-    // Candidates::push / Results::try_push rule (gpuHeap.h:23-36, 78-87).
-    // Returns how many were accepted.
-    __device__ __forceinline__ int push_items(int it_k, int it_id, int F, int lane) {
-        int accepted = 0;
+    // Append the first min(F, cap - sz) items (lanes 0..) while the set is not
+    // full, as one lane rotation; returns how many.
+    __device__ __forceinline__ int append_items(int it_k, int it_id, int F, int lane) {
         const int a = min(F, cap - sz);
         if (a > 0) {
             const int src = (lane - sz) & 63;
@@ -481,9 +486,16 @@ struct WSet {
             for (int s = 0; s < S; s++) d[s] = slot == s ? mk : d[s];
             if (lane < a) ids[sz + lane] = it_id;
             sz += a;
-            accepted = a;
             mx_ok = false;
         }
+        return a > 0 ? a : 0;
+    }
+    // Push items it_k/it_id held by lanes 0..F-1, in lane order, with the
+    // Candidates::push / Results::try_push rule (gpuHeap.h:23-36, 78-87).
+    // Returns how many were accepted.
+    __device__ __forceinline__ int push_items(int it_k, int it_id, int F, int lane) {
+        const int a = append_items(it_k, it_id, F, lane);
+        int accepted = a;
         if (F > a) {
             int M = cmax(lane);
             int start = a;
@@ -809,6 +821,13 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
     int next_eval = LEARN ? (P.L.min_hops > 0 ? P.L.min_hops : P.L.interval) : 0;
     int stop_votes = 0;
     int last_insert_hop = 0;
+    // feature scales (learned search only): 1 / distance scale, 1 / k, 1 / ef
+    float inv_sc = 0.f, inv_k = 0.f, inv_ef = 0.f;
+    if constexpr (LEARN) {
+        inv_sc = 1.0f / P.L.dist_scale;
+        inv_k = 1.0f / (float)k;
+        inv_ef = 1.0f / (float)ef;
+    }
 
     // Each expansion is gather (Bloom test-then-set of a row of ids in lane
     // order, distances of the fresh ones: lane r < F holds the r-th fresh id
@@ -842,8 +861,11 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
                                                shadow);
         }
         wave_lds_sync();
-        it_d = lane < F ? stage_d[lane] : INFINITY;
-        it_id = lane < F ? stage_id[lane] : -1;
+        // read unconditionally (64 staged entries), then keep the first F
+        const float sd = stage_d[lane];
+        const int si = stage_id[lane];
+        it_d = lane < F ? sd : INFINITY;
+        it_id = lane < F ? si : -1;
         it_k = dist_key<SK>(it_d);
         return F;
     };
@@ -926,11 +948,13 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
         if (cand.sz == 0) break;
         const int c_min = nx_key;
         const int p = cand.first_eq(c_min, lane);
+        // both ids in one LDS round trip: the popped node and the last entry, which fills its hole
         const int cn = cand.ids[p];
+        const int li = cand.ids[cand.sz - 1];
         int nb = nb_pref;
         uint32_t vp = vp_pref;
         if (cn != pf_node) load_row(cn, nb, vp);
-        cand.remove_at(p, lane);  // removing the minimum leaves the maximum unless the set empties
+        cand.remove_at(p, li, lane);  // removing the minimum leaves the maximum unless the set empties
         if (cand.sz == 0) cand.mx_ok = false;
         tr_expanded++;
         if (!small_res && res.sz == res.cap) {
@@ -956,20 +980,20 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
                 next_eval += P.L.late_hops > 0 && tr_expanded >= P.L.late_hops ? P.L.late_interval : P.L.interval;
             }
             if ((tracing || stopping) && tk.dk < INFINITY && due && tr_expanded >= P.L.min_hops) {
-                const float sc = P.L.dist_scale;
                 const int dkk = dist_key<SK>(tk.dk);
                 int closer = 0;  // frontier entries still closer than the k-th best
 #pragma unroll
                 for (int s = 0; s < S; s++) closer += __popcll(__ballot(cand.d[s] != KEY_EMPTY && cand.d[s] < dkk));
                 float f[LM_F];
+                // products with reciprocals rounded once per query (no division per consult)
                 f[0] = (float)tr_expanded * (1.0f / 64.0f);
-                f[1] = (key_dist<SK>(c_min) - tk.dk) / sc;
-                f[2] = (tk.dk - tk.d1) / sc;
+                f[1] = (key_dist<SK>(c_min) - tk.dk) * inv_sc;
+                f[2] = (tk.dk - tk.d1) * inv_sc;
                 f[3] = (float)(tr_expanded - last_insert_hop) * (1.0f / 16.0f);
-                f[4] = (float)closer / (float)k;
-                f[5] = (float)tk.inserts / (float)k;
-                f[6] = (float)cand.sz / (float)ef;
-                f[7] = dk_prev < INFINITY ? (dk_prev - tk.dk) / sc : 0.f;
+                f[4] = (float)closer * inv_k;
+                f[5] = (float)tk.inserts * inv_k;
+                f[6] = (float)cand.sz * inv_ef;
+                f[7] = dk_prev < INFINITY ? (dk_prev - tk.dk) * inv_sc : 0.f;
 #pragma unroll
                 for (int i = 0; i < LM_F; i++) f[i] = fminf(fmaxf(f[i], -8.f), 8.f);
                 dk_prev = tk.dk;

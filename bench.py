@@ -361,9 +361,14 @@ def main():
         (FETCH_SIZE / WRITE_SIZE scaled by their calibration)."""
         abytes = algorithmic_bytes(trace_rows, dim, block, args.k)
         achieved = abytes / (kernel_ms / 1e3) / 1e9
+        expansions = int(trace_rows[:, 2].astype(np.int64).sum())
         out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kname,
-               "kernel_avg_ms": round(kernel_ms, 4), "algorithmic_bytes_per_launch": abytes}
+               "kernel_avg_ms": round(kernel_ms, 4), "algorithmic_bytes_per_launch": abytes,
+               "expansions_per_launch": expansions,
+               # the index's visit table: the popped node's 2 x block filter-position words per expansion
+               # (read beside the algorithm's bytes, not counted in them)
+               "visit_table_bytes_per_launch": expansions * 8 * block}
         for fname, field in (("traffic.json", "traffic"), ("kernel_profiles.json", "rocprof")):
             path = os.path.join(ROOT, "profiles", fname)
             rec = json.load(open(path)).get(key) if os.path.exists(path) else None

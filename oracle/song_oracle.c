@@ -542,18 +542,20 @@ static void search_one(const or_space *sp, const or_params *p, or_work *w, const
             }
             const float dk = topk_dk(&w->topk);
             if (dk < INFINITY && due && hop >= st->min_hops) {
-                const float sc = st->dist_scale, d1 = w->topk.v[0];
+                /* reciprocals rounded once, then each feature one product (the kernel's order) */
+                const float isc = 1.0f / st->dist_scale, d1 = w->topk.v[0];
+                const float ik = 1.0f / (float)p->k, ief = 1.0f / (float)p->ef;
                 int closer = 0; /* frontier entries (the popped one removed) closer than the k-th best */
                 for (int i = 0; i < w->cand.sz; i++) closer += w->cand.d[i] < dk;
                 float f[8];
                 f[0] = (float)hop * (1.0f / 64.0f);
-                f[1] = (c_min - dk) / sc;
-                f[2] = (dk - d1) / sc;
+                f[1] = (c_min - dk) * isc;
+                f[2] = (dk - d1) * isc;
                 f[3] = (float)(hop - w->last_insert_hop) * (1.0f / 16.0f);
-                f[4] = (float)closer / (float)p->k;
-                f[5] = (float)w->topk.inserts / (float)p->k;
-                f[6] = (float)w->cand.sz / (float)p->ef;
-                f[7] = dk_prev < INFINITY ? (dk_prev - dk) / sc : 0.f;
+                f[4] = (float)closer * ik;
+                f[5] = (float)w->topk.inserts * ik;
+                f[6] = (float)w->cand.sz * ief;
+                f[7] = dk_prev < INFINITY ? (dk_prev - dk) * isc : 0.f;
                 for (int i = 0; i < 8; i++) f[i] = clamp8(f[i]);
                 dk_prev = dk;
                 w->topk.inserts = 0;
