@@ -315,7 +315,8 @@ def fit(index: Index, base, train_queries, *, k=10, ef=256, n_seeds=4, bloom_bit
     if cost_search:
         copts = {kk: v for kk, v in cost_search.items() if kk not in ("ef", "mode", "waves_per_block")}
         copts["learned"] = copts.get("learned", 2) & 3  # the labelling run itself keeps input order
-    r = index.search(train_queries, k, cost_ef or ef, bloom_bits=bloom_bits, n_seeds=n_seeds, **copts)
+    sopts = {"bloom_bits": bloom_bits, "n_seeds": n_seeds, **copts}  # cost_search's own values win
+    r = index.search(train_queries, k, cost_ef or ef, **sopts)
     hops = np.log2(np.maximum(r.trace[:, 2], 1).astype(np.float64))
     mean, std = hops.mean(), max(hops.std(), 1e-6)
     target = ((hops - mean) / std).astype(np.float32)
