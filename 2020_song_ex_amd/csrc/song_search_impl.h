@@ -1127,17 +1127,17 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
 template <int S>
 __host__ __device__ constexpr size_t wave_lds_u64(int bloom_words) { return (size_t)bloom_words + 64 + 64 * S; }
 
-// Register budget: with two 16-byte chunks per lane the plain search fits in
-// 80 VGPRs, so 6 waves share a SIMD (4 at the compiler's own choice).  The
-// search with the learned stop is held to the same 80: a few per-query values
-// spill (none inside the persistent kernel's hop loop), and it runs 5 % faster
-// than at 96 VGPRs (5 waves).  (The plain search at 72 VGPRs, 7 waves, spills inside its hop
-// loop and runs slower.)  With one set slot per lane (queue <= 64) the learned
-// search goes to 72 VGPRs, 7 waves: 2 % faster than 6, and 8 waves (64 VGPRs)
-// gain nothing more.  Larger rows and the feature-recording search keep the
+// Register budget: with two 16-byte chunks per lane and the wave's LDS base
+// addresses in scalar registers, the plain search fits in 72 VGPRs without a
+// spill, so 7 waves share a SIMD (3 % faster than 6 at 80 VGPRs; before the
+// LDS bases went scalar it needed 80 with spills).  The search with the
+// learned stop is held to 80 with more set slots (6 waves), and with one set
+// slot per lane (queue <= 64) to 72 (7 waves): 2 % faster than 6; 8 waves
+// (64 VGPRs) gained nothing more.  A few of its per-query values spill, none
+// inside the hop loop.  Larger rows and the feature-recording search keep the
 // compiler's own choice.
 __host__ __device__ constexpr int waves_per_simd(int qch, int learn, int s) {
-    return qch > 2 ? 1 : learn == 0 ? 6 : learn == 1 ? (s == 1 ? 7 : 6) : 1;
+    return qch > 2 ? 1 : learn == 0 ? 7 : learn == 1 ? (s == 1 ? 7 : 6) : 1;
 }
 
 // One wave per query (the reference's one block per query, kernalGraph.h:354).
@@ -1147,7 +1147,10 @@ search_kernel(SearchArgs P) {
     extern __shared__ unsigned long long smem[];
     unsigned long long* waves_lds = smem;
     const float* stop_lds = LEARN == 1 ? P.L.s_weights : nullptr;
-    const int wave = threadIdx.x >> 6;
+    // the wave's index is the same in every lane: read it once so the wave's LDS base
+    // addresses are scalar (derived from threadIdx.x they would count as varying and take
+    // vector registers)
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int pos = blockIdx.x * (blockDim.x >> 6) + wave;
     if (pos >= P.nq) return;
     const int qid = P.order ? P.order[pos] : pos;
@@ -1200,7 +1203,10 @@ persistent_kernel(SearchArgs P) {
     extern __shared__ unsigned long long smem[];
     unsigned long long* waves_lds = smem;
     const float* stop_lds = LEARN == 1 ? P.L.s_weights : nullptr;
-    const int wave = threadIdx.x >> 6;
+    // the wave's index is the same in every lane: read it once so the wave's LDS base
+    // addresses are scalar (derived from threadIdx.x they would count as varying and take
+    // vector registers)
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = lane_id();
     unsigned long long* lds = waves_lds + (size_t)wave * wave_lds_u64<S>(P.bloom_words);
     if constexpr (kResidentServing) {
