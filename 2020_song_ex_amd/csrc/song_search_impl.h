@@ -344,7 +344,7 @@ __device__ __forceinline__ uint32_t wave_min_u(uint32_t v) {
 }
 
 // Unsorted bounded set of (dist, id): the key of entry i in VGPR slot i/64 of
-// lane i%64, its id in LDS at ids[i].  Every scan (min, max, first index of a
+// lane i%64, its id in LDS at ids[i] (in a VGPR when S = 1).  Every scan (min, max, first index of a
 // key) runs on the keys; ids are only read or written at one known index, so
 // they cost one LDS access instead of a select per slot.  Slots at or beyond
 // sz hold KEY_EMPTY, which no scan picks and == never matches, so scans need
@@ -356,7 +356,12 @@ struct WSet {
     // per-slot selects folded back into an indexed scratch access.
     typedef int kvec __attribute__((ext_vector_type(S == 1 ? 2 : S)));
     kvec d;
-    int* ids;  // this wave's LDS array of 64 * S ids
+    // Ids: with one slot per lane (S = 1) entry i's id sits in lane i of idv,
+    // read with one readlane and written with one select (no LDS round trip);
+    // with more slots in this wave's LDS array of 64 * S ids.
+    static constexpr bool VID = S == 1;
+    int idv;
+    int* ids;
     int sz;    // wave-uniform
     int cap;   // wave-uniform
     int mx;    // cached maximum key, valid when mx_ok
@@ -364,6 +369,7 @@ struct WSet {
 
     __device__ __forceinline__ void init(int c, int* lds_ids) {
         ids = lds_ids;
+        idv = -1;
         sz = 0;
         cap = c < 1 ? 1 : c;
         mx_ok = false;
@@ -425,9 +431,18 @@ struct WSet {
         }
         return r;
     }
+    // Id of entry idx (a wave-uniform index).
+    __device__ __forceinline__ int get_id(int idx) const {
+        if constexpr (VID) return __builtin_amdgcn_readlane(idv, idx);
+        else return ids[idx];
+    }
+    __device__ __forceinline__ void set_id(int idx, int iv, int lane) {
+        if constexpr (VID) idv = lane == idx ? iv : idv;
+        else if (lane == 0) ids[idx] = iv;
+    }
     __device__ __forceinline__ void get(int idx, int& kv, int& iv) const {
         kv = get_d(idx);
-        iv = ids[idx];
+        iv = get_id(idx);
     }
     // Only lane idx%64 of slot idx/64 changes; one lane writes the id.
     __device__ __forceinline__ void put_d(int idx, int kv, int lane) {
@@ -443,11 +458,11 @@ struct WSet {
     }
     __device__ __forceinline__ void put(int idx, int kv, int iv, int lane) {
         put_d(idx, kv, lane);
-        if (lane == 0) ids[idx] = iv;
+        set_id(idx, iv, lane);
     }
     // Remove entry p, filling the hole with the last entry (gpuHeap.h:44-48);
     // the vacated last slot becomes empty.  The caller keeps mx_ok as fits.
-    __device__ __forceinline__ void remove_at(int p, int lane) { remove_at(p, ids[sz - 1], lane); }
+    __device__ __forceinline__ void remove_at(int p, int lane) { remove_at(p, get_id(sz - 1), lane); }
     // The same with the last entry's id already read (li = ids[sz - 1]).
     __device__ __forceinline__ void remove_at(int p, int li, int lane) {
         const int last = sz - 1;
@@ -468,7 +483,7 @@ struct WSet {
                 if (b) {
                     const int l = ffs64(b);
                     d[s] = lane == l ? kv : d[s];
-                    if (lane == 0) ids[s * 64 + l] = iv;
+                    set_id(s * 64 + l, iv, lane);
                     done = true;
                 }
             }
@@ -484,7 +499,12 @@ struct WSet {
             const int slot = src < a ? (sz + src) >> 6 : -1;
 #pragma unroll
             for (int s = 0; s < S; s++) d[s] = slot == s ? mk : d[s];
-            if (lane < a) ids[sz + lane] = it_id;
+            if constexpr (VID) {
+                const int mi = __shfl(it_id, src, 64);
+                idv = src < a ? mi : idv;
+            } else {
+                if (lane < a) ids[sz + lane] = it_id;
+            }
             sz += a;
             mx_ok = false;
         }
@@ -949,8 +969,8 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
         const int c_min = nx_key;
         const int p = cand.first_eq(c_min, lane);
         // both ids in one LDS round trip: the popped node and the last entry, which fills its hole
-        const int cn = cand.ids[p];
-        const int li = cand.ids[cand.sz - 1];
+        const int cn = cand.get_id(p);
+        const int li = cand.get_id(cand.sz - 1);
         int nb = nb_pref;
         uint32_t vp = vp_pref;
         if (cn != pf_node) load_row(cn, nb, vp);
@@ -1025,7 +1045,7 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
         const int F = gather_row(nb, nb >= 0, P.block, use_vpos, vp, [&]() __attribute__((always_inline)) {
             if (old_ok) {
                 old_min = cand.vmin(lane);
-                old_id = cand.ids[cand.first_eq(old_min, lane)];
+                old_id = cand.get_id(cand.first_eq(old_min, lane));
             }
         });
         tr_visited += F;
@@ -1061,7 +1081,7 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
                 fast = false;
                 break;
             }
-            const int iv = res.ids[p];
+            const int iv = res.get_id(p);
             if (lane == i) {
                 od = m;
                 oi = iv;
