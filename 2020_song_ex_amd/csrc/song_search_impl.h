@@ -434,7 +434,9 @@ struct WSet {
     // Id of entry idx (a wave-uniform index).
     __device__ __forceinline__ int get_id(int idx) const {
         if constexpr (VID) return __builtin_amdgcn_readlane(idv, idx);
-        else return ids[idx];
+        // every lane reads the same word: readfirstlane makes the value scalar, so the
+        // branches and row addresses that use it stay scalar too
+        else return __builtin_amdgcn_readfirstlane(ids[idx]);
     }
     __device__ __forceinline__ void set_id(int idx, int iv, int lane) {
         if constexpr (VID) idv = lane == idx ? iv : idv;
