@@ -147,7 +147,7 @@ int make_plan(const song_index* ix, int32_t nq, int32_t k, int32_t ef, const son
     if (ef < k) ef = k;  // kernalGraph.h:315
     if (ef > 512) return fail(SONG_ERR_ARG, "ef (queue size) must be <= 512 after ef = max(ef, k)");
     p->ef = ef;
-    p->S = ef <= 64 ? 1 : ef <= 128 ? 2 : ef <= 192 ? 3 : ef <= 256 ? 4 : ef <= 384 ? 6 : ef <= 448 ? 7 : 8;
+    p->S = (ef + 63) / 64;  // set slots per lane: 1..8
     int64_t bits = o->bloom_bits ? o->bloom_bits : 16384;
     if (!is_pow2(bits) || bits < 512 || bits > 65536)
         return fail(SONG_ERR_ARG, "bloom_bits must be a power of two in [512, 65536]");

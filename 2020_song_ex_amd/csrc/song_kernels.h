@@ -138,7 +138,7 @@ hipError_t launch_learned_forward(const LearnedArgs& L, const PlanArgs& P, int w
 // by predicted cost, most expensive first (counting sort over 256 bins).
 hipError_t launch_plan(const PlanArgs& P, int metric, int32_t* order, hipStream_t st);
 
-// metric: METRIC_*; S: set slots per lane (cap <= 64*S, S in {1,2,3,4,6,7,8});
+// metric: METRIC_*; S: set slots per lane (cap <= 64*S, S = 1..8);
 // G: lanes per row (power of two); qch: 16-byte chunks per lane (G*qch*4 >= dstride).
 hipError_t launch_search(const SearchArgs& a, int metric, int S, int G, int qch, const LaunchShape& shape,
                          hipStream_t st);

@@ -12,6 +12,7 @@ namespace song {
     hipError_t launch_search_part_##M##_2(const SearchArgs&, int, int, const LaunchShape&, hipStream_t);       \
     hipError_t launch_search_part_##M##_3(const SearchArgs&, int, int, const LaunchShape&, hipStream_t);       \
     hipError_t launch_search_part_##M##_4(const SearchArgs&, int, int, const LaunchShape&, hipStream_t);       \
+    hipError_t launch_search_part_##M##_5(const SearchArgs&, int, int, const LaunchShape&, hipStream_t);       \
     hipError_t launch_search_part_##M##_6(const SearchArgs&, int, int, const LaunchShape&, hipStream_t);       \
     hipError_t launch_search_part_##M##_7(const SearchArgs&, int, int, const LaunchShape&, hipStream_t);       \
     hipError_t launch_search_part_##M##_8(const SearchArgs&, int, int, const LaunchShape&, hipStream_t);       \
@@ -22,6 +23,7 @@ namespace song {
         case 2: return launch_search_part_##M##_2(a, G, qch, ls, st);                                          \
         case 3: return launch_search_part_##M##_3(a, G, qch, ls, st);                                          \
         case 4: return launch_search_part_##M##_4(a, G, qch, ls, st);                                          \
+        case 5: return launch_search_part_##M##_5(a, G, qch, ls, st);                                          \
         case 6: return launch_search_part_##M##_6(a, G, qch, ls, st);                                          \
         case 7: return launch_search_part_##M##_7(a, G, qch, ls, st);                                          \
         case 8: return launch_search_part_##M##_8(a, G, qch, ls, st);                                          \

@@ -61,7 +61,8 @@ def test_config1_exact(song, oracle, c1):
 
 
 @pytest.mark.parametrize("k,ef", [(1, 1), (10, 10), (10, 32), (10, 50), (10, 100), (10, 150), (10, 200),
-                                  (100, 100), (100, 256), (10, 300), (100, 384), (10, 448), (10, 512), (64, 64), (65, 65)])
+                                  (100, 100), (100, 256), (10, 257), (10, 300), (10, 320), (10, 321), (100, 384), (10, 448),
+                                  (10, 512), (64, 64), (65, 65)])
 def test_queue_sizes(song, oracle, c1, k, ef):
     cfg, adj = c1
     r, o = run_both(song, oracle, cfg["base"], adj, cfg["queries"][:60], k, ef)
