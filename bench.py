@@ -343,9 +343,7 @@ def main():
     plain_host = host_buffers(args.k, ef, **plain_params)
     log(f"plain through host buffers (query H2D + kernel + ids D2H): {plain_host}")
 
-    def slots_for(q):
-        return (1 if q <= 64 else 2 if q <= 128 else 3 if q <= 192 else 4 if q <= 256 else 6 if q <= 384
-                else 7 if q <= 448 else 8)
+    slots_for = _slots
 
     # lanes per row and 16-byte chunks per lane, as the library picks them for dim
     d4 = (dim + 3) // 4
@@ -656,7 +654,9 @@ def config1_leg(song, datasets, args, threads):
 # ---------------------------------------------------------------- the other workloads
 
 def _slots(q):
-    return 1 if q <= 64 else 2 if q <= 128 else 3 if q <= 192 else 4 if q <= 256 else 6 if q <= 384 else 7 if q <= 448 else 8
+    """Set slots per lane of a queue of q entries, as the library picks them (the kernels'
+    template argument S)."""
+    return (q + 63) // 64
 
 
 def _timed_device(idx, dq, nq, k, ef, d_ids, args, **kw):
