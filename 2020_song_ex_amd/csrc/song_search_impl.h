@@ -33,6 +33,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "song_kernels.h"
 
 namespace song {
@@ -1285,9 +1287,12 @@ static hipError_t launch_impl(const SearchArgs& a, const LaunchShape& ls, hipStr
         hipLaunchKernelGGL((persistent_kernel<METRIC, S, G, QCH, LEARN>), dim3(ls.persistent_blocks), dim3(threads),
                            lds, st, a);
     } else if (a.queue_head) {
+        // no more workgroups than the queries need: with fewer queries than the grid holds, the
+        // idle waves would only share SIMDs with the working ones
+        const int need = (a.nq + ls.waves_per_block - 1) / ls.waves_per_block;
+        const int blocks = std::max(1, std::min(ls.persistent_blocks, need));
         hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(64), 0, st, a.queue_head);
-        hipLaunchKernelGGL((persistent_kernel<METRIC, S, G, QCH, LEARN>), dim3(ls.persistent_blocks), dim3(threads),
-                           lds, st, a);
+        hipLaunchKernelGGL((persistent_kernel<METRIC, S, G, QCH, LEARN>), dim3(blocks), dim3(threads), lds, st, a);
     } else {
         const int blocks = (a.nq + ls.waves_per_block - 1) / ls.waves_per_block;
         hipLaunchKernelGGL((search_kernel<METRIC, S, G, QCH, LEARN>), dim3(blocks), dim3(threads), lds, st, a);
