@@ -318,8 +318,8 @@ def main():
         log(f"plain configuration from {tuned_path}: queue {ef} {plain_params}")
     if not ef and extra is not None:
         # the visited filter's size is a knob of both methods (the reference's default is 2^14 bits)
-        grid = {"ef": [320, 336, 352, 368, 384, 416, 448, 512], "mode": [0], "waves_per_block": [1, 4],
-                "bloom_bits": [1 << 14, 1 << 15]}
+        grid = {"ef": [256, 272, 288, 304, 320, 336, 352, 368, 384, 416, 448, 512], "mode": [0],
+                "waves_per_block": [1, 4], "bloom_bits": [1 << 14, 1 << 15]}
         res = autotune.tune(idx, valid, vgt, args.k, grid, tune_target, log=log)
         plain_tuned = res.table()
         if res.best is not None:
