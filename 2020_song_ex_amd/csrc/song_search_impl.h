@@ -1158,8 +1158,12 @@ __host__ __device__ constexpr size_t wave_lds_u64(int bloom_words) { return (siz
 // learned stop is held to 80 with more set slots (6 waves), and with one set
 // slot per lane (queue <= 64) to 72 (7 waves): 2 % faster than 6; 8 waves
 // (64 VGPRs) gained nothing more.  A few of its per-query values spill, none
-// inside the hop loop.  Larger rows and the feature-recording search keep the
-// compiler's own choice.
+// inside the hop loop.  Four or more 16-byte chunks per lane (rows longer than
+// 128 floats at 16 lanes, or 8 lanes per 128-float row) and the
+// feature-recording search keep the compiler's own choice: with 8 lanes per
+// 128-float row the plain search takes 94 VGPRs (5 waves per SIMD) and runs
+// 2-3 % faster than at 16 lanes; held to 80 or 72 VGPRs it spilled and ran 11 %
+// and 39 % slower.
 __host__ __device__ constexpr int waves_per_simd(int qch, int learn, int s) {
     return qch > 2 ? 1 : learn == 0 ? 7 : learn == 1 ? (s == 1 ? 7 : 6) : 1;
 }

@@ -318,8 +318,10 @@ def main():
         log(f"plain configuration from {tuned_path}: queue {ef} {plain_params}")
     if not ef and extra is not None:
         # the visited filter's size is a knob of both methods (the reference's default is 2^14 bits)
+        # (8 lanes per 128-float row: one distance pass for the ~4.4 fresh rows of a hop, at the
+        # compiler's register choice, 5 waves per SIMD)
         grid = {"ef": [256, 272, 288, 304, 320, 336, 352, 368, 384, 416, 448, 512], "mode": [0],
-                "waves_per_block": [1, 4], "bloom_bits": [1 << 14, 1 << 15]}
+                "waves_per_block": [1, 4], "bloom_bits": [1 << 14, 1 << 15], "lanes_per_row": [0, 8]}
         res = autotune.tune(idx, valid, vgt, args.k, grid, tune_target, log=log)
         plain_tuned = res.table()
         if res.best is not None:
@@ -345,14 +347,20 @@ def main():
 
     slots_for = _slots
 
-    # lanes per row and 16-byte chunks per lane, as the library picks them for dim
+    # lanes per row and 16-byte chunks per lane, as the library picks them for dim (or for the
+    # given lanes_per_row): the kernels' template arguments G and QCH
     d4 = (dim + 3) // 4
-    lanes = 2
-    while lanes < 64 and lanes * 2 < d4:
-        lanes *= 2
-    chunks = 2
-    while lanes * chunks < d4:
-        chunks *= 2
+
+    def shape_of(params):
+        lanes = params.get("lanes_per_row") or 0
+        if not lanes:
+            lanes = 2
+            while lanes < 64 and lanes * 2 < d4:
+                lanes *= 2
+        chunks = 2
+        while lanes * chunks < d4:
+            chunks *= 2
+        return lanes, chunks
 
     def roofline_of(trace_rows, kernel_ms, kname, key):
         """Algorithmic bytes of one launch / its HIP-event duration.  Beside it, from the committed
@@ -381,6 +389,7 @@ def main():
                 out["rocprof_summary"] = rec["summary"]
         return out
 
+    lanes, chunks = shape_of(plain_params)
     pname = f"search_kernel<0,{slots_for(ef)},{lanes},{chunks},0>"
     plain_roofline = roofline_of(trace, plain_kernel_ms, pname,
                                  f"{pname} {args.n}x{dim} nq={nq} queue={ef} k={args.k} "
@@ -443,6 +452,7 @@ def main():
             lqps, lms_step, lkernel_ms, lkernel_std = timed(args.k, lef, **p)
             lhost = host_buffers(args.k, lef, **p)
             best_search = (lef, p)
+            lanes, chunks = shape_of(p)
             lname = f"persistent_kernel<0,{slots_for(lef)},{lanes},{chunks},1>"
             lkey = f"{lname} {args.n}x{dim} nq={nq} " + " ".join(f"{kk}={v}" for kk, v in sorted({"ef": lef, **p}.items()))
             learned_roofline = roofline_of(ltr, lkernel_ms, lname, lkey)

@@ -3,8 +3,9 @@
 on size, checked where the timed numbers come from.
 
 * A sample of queries against the oracle, bit for bit, at the tuned plain
-  queue (320) with its 2^15-bit visited filter, and at the reference's
-  2^14 bits: ids, distances and the reference's trace counters.
+  queue (320) with its 2^15-bit visited filter (16 and 8 lanes per row), and
+  at the reference's 2^14 bits: ids, distances and the reference's trace
+  counters.
 * The persistent kernel against the plain one on all 10,000 queries.
 * The learned search's result cap with a stop that never fires, against the
   oracle's restatement of that option.
@@ -44,10 +45,10 @@ def recall(ids, gt):
 def test_sample_matches_oracle(full, oracle):
     base, q, adj, idx, gt, _ = full
     sample = q[::50]  # 200 queries
-    for ef, bits in ((320, 1 << 15), (352, 1 << 14)):
-        r = idx.search(sample, 10, ef, bloom_bits=bits)
-        # 16 lanes per 128-float row; integer data sums exactly in any order
-        oi, od, ot = oracle.search(base, adj, sample, 10, ef, block=32, bloom_bits=bits, group=16)
+    for ef, bits, lanes in ((320, 1 << 15, 16), (320, 1 << 15, 8), (352, 1 << 14, 16)):
+        r = idx.search(sample, 10, ef, bloom_bits=bits, lanes_per_row=lanes)
+        # 16 (the default) or 8 lanes per 128-float row; integer data sums exactly in any order
+        oi, od, ot = oracle.search(base, adj, sample, 10, ef, block=32, bloom_bits=bits, group=lanes)
         np.testing.assert_array_equal(r.ids, oi)
         np.testing.assert_array_equal(r.dists.view(np.uint32), od.view(np.uint32))
         np.testing.assert_array_equal(r.trace, ot)
