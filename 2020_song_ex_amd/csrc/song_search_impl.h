@@ -1,7 +1,8 @@
 #pragma once
 // song_search_impl.h -- the SONG graph-search kernels for MI355X (gfx950).
-// Included by one translation unit per metric (song_search_l2.hip, ..._ip,
-// ..._hamming), which instantiate the launch templates for that metric.
+// Included by song_search_part.hip, compiled once per metric and set size
+// (slots per lane), which instantiates the launch templates of that pair, and
+// by song_search.hip (dispatch and the visit-table kernel).
 //
 // One 64-lane wavefront searches one query.  The algorithm is the reference's
 // kg_search_kernel / hkg_search_kernel (kernalGraph.h:184-299,
