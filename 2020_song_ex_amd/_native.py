@@ -40,7 +40,7 @@ class SongSearchOpts(ctypes.Structure):
                 ("waves_per_cu", _I32), ("learned", _I32), ("stop_threshold", ctypes.c_float),
                 ("stop_min_hops", _I32), ("stop_interval", _I32), ("stop_max_hops", _I32),
                 ("stop_confirm", _I32), ("result_cap", _I32), ("stop_late_hops", _I32),
-                ("stop_late_interval", _I32)]
+                ("stop_late_interval", _I32), ("entry_late_hops", _I32)]
 
 
 EXPORTS = (
@@ -129,8 +129,10 @@ MODE_PLAIN, MODE_PERSISTENT = 0, 1
 
 def make_opts(bloom_bits=0, batch=0, n_seeds=0, waves_per_block=0, lanes_per_row=0, mode=0,
               waves_per_cu=0, learned=0, stop_threshold=0.0, stop_min_hops=0, stop_interval=1,
-              stop_max_hops=0, stop_confirm=0, result_cap=0, stop_late_hops=0, stop_late_interval=0):
+              stop_max_hops=0, stop_confirm=0, result_cap=0, stop_late_hops=0, stop_late_interval=0,
+              entry_late_hops=0):
     o = SongSearchOpts()
+    o.entry_late_hops = entry_late_hops
     o.result_cap = result_cap
     o.stop_late_hops = stop_late_hops
     o.stop_late_interval = stop_late_interval

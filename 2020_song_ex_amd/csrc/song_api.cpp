@@ -135,6 +135,7 @@ struct Plan {
     float stop_threshold;
     int stop_min_hops, stop_interval, stop_max_hops, stop_confirm, result_cap;
     int stop_late_hops, stop_late_interval;
+    int entry_late_hops;
     song::LaunchShape shape;
 };
 
@@ -211,6 +212,10 @@ int make_plan(const song_index* ix, int32_t nq, int32_t k, int32_t ef, const son
     if (o->stop_late_hops < 0) return fail(SONG_ERR_ARG, "stop_late_hops must be >= 0");
     if (o->stop_late_hops > 0 && o->stop_late_interval < 1)
         return fail(SONG_ERR_ARG, "stop_late_interval must be >= 1 when stop_late_hops > 0");
+    p->entry_late_hops = o->entry_late_hops;
+    if (o->entry_late_hops < 0) return fail(SONG_ERR_ARG, "entry_late_hops must be >= 0");
+    if (o->entry_late_hops > 0 && ((o->learned & 3) != 3 || (o->learned & 8)))
+        return fail(SONG_ERR_ARG, "entry_late_hops needs learned bits 0 and 1 without bit 3");
     if (o->learned) {
         if (!ix->has_model) return fail(SONG_ERR_STATE, "learned search needs a model (song_index_set_model)");
         if (k > 64) return fail(SONG_ERR_ARG, "learned search supports k <= 64");
@@ -260,6 +265,7 @@ song::SearchArgs make_args(const song_index* ix, const Plan& p, const void* dq, 
         a.L.max_hops = p.stop_max_hops;
         a.L.confirm = p.stop_confirm;
         a.L.result_cap = p.result_cap;
+        a.L.entry_late_hops = p.entry_late_hops;
         a.L.trace = nullptr;
         a.L.trace_count = nullptr;
         a.L.final_dk = nullptr;

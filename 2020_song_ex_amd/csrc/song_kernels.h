@@ -66,6 +66,7 @@ struct LearnedArgs {
     int32_t max_hops;         // stop after this many expansions (0 = no cap)
     int32_t confirm;          // consecutive consults above the threshold needed to stop (>= 1)
     int32_t result_cap;       // result-set capacity below ef, without the reference's early stop (0 = ef)
+    int32_t entry_late_hops;  // > 0: start at the reference seeds, the entry list joins at this hop
     // training traces (all null outside trace mode)
     float* trace;             // [nq x max_evals x (LM_F + 1)]: features, k-th best distance
     int32_t* trace_count;     // [nq]

@@ -823,7 +823,11 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
     // compares with the worst of ef results -- is left to the stop network
     // and the hop cap (the host requires one).  The traversal (candidates,
     // Bloom) does not depend on the result set, and its k best are the same.
-    const bool small_res = LEARN == 1 && P.L.result_cap > 0 && P.L.result_cap < ef;
+    // LEARN 1: the learned stop; 3: the same with a late entry (its own
+    // instantiation, so the plain learned search keeps its registers);
+    // 2: record the stop's features instead
+    constexpr bool STOPL = LEARN == 1 || LEARN == 3;
+    const bool small_res = STOPL && P.L.result_cap > 0 && P.L.result_cap < ef;
     cand.init(ef, set_ids);
     res.init(small_res ? max(P.L.result_cap, k) : ef, set_ids + 64 * S);
     // QueryTrace, telemetry.h:5-18 (wave-uniform counters)
@@ -836,7 +840,7 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
     // learned termination state (k <= 64 on this path)
     // LEARN 1: the learned stop; LEARN 2: record its features instead
     const bool tracing = LEARN == 2 && P.L.trace != nullptr;
-    const bool stopping = LEARN == 1 && P.L.use_stop;
+    const bool stopping = STOPL && P.L.use_stop;
     TopK tk;
     tk.init(k);
     float dk_prev = INFINITY;
@@ -948,13 +952,16 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
     };
 
     // Seeds: kernalGraph.h:215-231 -- ids (size_t)p * n / n_seeds, or the
-    // caller's (or the planning pass's) entry list.
+    // caller's (or the planning pass's) entry list; with a late entry the
+    // reference seeds first and the entry list at hop entry_late_hops.
+    bool late_entry = false;
+    if constexpr (LEARN == 3) late_entry = P.L.entry_late_hops > 0 && P.entries != nullptr;
     {
         const int e = P.n_entry <= 32 ? (lane & 31) : lane;
         int sid = 0;
         bool sv = e < P.n_entry;
         if (sv) {
-            if (P.entries) {
+            if (P.entries && !late_entry) {
                 sid = P.entries[(size_t)qid * P.n_entry + e];
                 sv = sid >= 0 && sid < P.n;
             } else {
@@ -1041,6 +1048,19 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
                         break;
                     }
                 }
+            }
+        }
+        // the entry list joins (late entry): tested and pushed like a row,
+        // before the popped node's row, whose next-pop prediction then sees it
+        if constexpr (LEARN == 3) {
+            if (late_entry && tr_expanded == P.L.entry_late_hops) {
+                const int e = P.n_entry <= 32 ? (lane & 31) : lane;
+                bool sv = e < P.n_entry;
+                const int sid = sv ? P.entries[(size_t)qid * P.n_entry + e] : 0;
+                sv = sv && sid >= 0 && sid < P.n;
+                const int Fe = gather_row(sid, sv, P.n_entry, false, 0u, []() {});
+                tr_visited += Fe;
+                push_row(Fe);
             }
         }
         // the candidates' minimum without cn, for the next prediction, found
@@ -1166,7 +1186,7 @@ __host__ __device__ constexpr size_t wave_lds_u64(int bloom_words) { return (siz
 // 2-3 % faster than at 16 lanes; held to 80 or 72 VGPRs it spilled and ran 11 %
 // and 39 % slower.
 __host__ __device__ constexpr int waves_per_simd(int qch, int learn, int s) {
-    return qch > 2 ? 1 : learn == 0 ? 7 : learn == 1 ? (s == 1 ? 7 : 6) : 1;
+    return qch > 2 ? 1 : learn == 0 ? 7 : learn == 1 || learn == 3 ? (s == 1 ? 7 : 6) : 1;
 }
 
 // One wave per query (the reference's one block per query, kernalGraph.h:354).
@@ -1175,7 +1195,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(waves_
 search_kernel(SearchArgs P) {
     extern __shared__ unsigned long long smem[];
     unsigned long long* waves_lds = smem;
-    const float* stop_lds = LEARN == 1 ? P.L.s_weights : nullptr;
+    const float* stop_lds = LEARN == 1 || LEARN == 3 ? P.L.s_weights : nullptr;
     // the wave's index is the same in every lane: read it once so the wave's LDS base
     // addresses are scalar (derived from threadIdx.x they would count as varying and take
     // vector registers)
@@ -1231,7 +1251,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(waves_
 persistent_kernel(SearchArgs P) {
     extern __shared__ unsigned long long smem[];
     unsigned long long* waves_lds = smem;
-    const float* stop_lds = LEARN == 1 ? P.L.s_weights : nullptr;
+    const float* stop_lds = LEARN == 1 || LEARN == 3 ? P.L.s_weights : nullptr;
     // the wave's index is the same in every lane: read it once so the wave's LDS base
     // addresses are scalar (derived from threadIdx.x they would count as varying and take
     // vector registers)
@@ -1312,6 +1332,7 @@ static hipError_t launch_one(const SearchArgs& a, const LaunchShape& ls, hipStre
     if (!a.learned) return launch_impl<METRIC, S, G, QCH, 0>(a, ls, st);
     if constexpr (QCH <= 4) {
         if (a.L.trace) return launch_impl<METRIC, S, G, QCH, 2>(a, ls, st);
+        if (a.L.entry_late_hops > 0 && a.entries) return launch_impl<METRIC, S, G, QCH, 3>(a, ls, st);
         return launch_impl<METRIC, S, G, QCH, 1>(a, ls, st);
     }
     return hipErrorInvalidValue;

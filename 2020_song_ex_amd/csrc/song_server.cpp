@@ -480,6 +480,7 @@ int song_server_create(song_index* ix, const song_server_opts* o, song_server** 
         plain_opts.stop_interval = 1;
         plain_opts.stop_late_hops = plain_opts.stop_late_interval = 0;
         plain_opts.result_cap = 0;
+        plain_opts.entry_late_hops = 0;
         if ((rc = song::check_search_opts(ix, fallback_k, opts.ef_max, &opts.search)) != SONG_OK) return rc;
         if ((rc = song::check_search_opts(ix, opts.k_max, opts.ef_max, &plain_opts)) != SONG_OK) return rc;
     } else if ((rc = song::check_search_opts(ix, opts.k_max, opts.ef_max, &opts.search)) != SONG_OK) {

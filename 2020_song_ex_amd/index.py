@@ -191,7 +191,7 @@ class Index:
     def search(self, queries, k, ef, *, bloom_bits=0, batch=0, n_seeds=0, waves_per_block=0,
                lanes_per_row=0, mode=0, waves_per_cu=0, learned=0, stop_threshold=0.0, stop_min_hops=0,
                stop_interval=1, stop_max_hops=0, stop_confirm=0, result_cap=0, stop_late_hops=0, stop_late_interval=0,
-               dists=True, trace=True) -> SearchResult:
+               entry_late_hops=0, dists=True, trace=True) -> SearchResult:
         """gpu_search (kernalGraph.h:307-382): host queries in, best-first ids out."""
         q = self._queries(queries)
         nq = q.shape[0]
@@ -200,7 +200,7 @@ class Index:
         tr = np.empty((nq, 12), np.int32) if trace else None
         o = N.make_opts(bloom_bits, batch, n_seeds, waves_per_block, lanes_per_row, mode, waves_per_cu,
                         learned, stop_threshold, stop_min_hops, stop_interval, stop_max_hops, stop_confirm,
-                        result_cap, stop_late_hops, stop_late_interval)
+                        result_cap, stop_late_hops, stop_late_interval, entry_late_hops)
         tm = N.SongTiming()
         bsz = batch if batch > 0 else max(nq, 1)
         nb = max(1, -(-nq // bsz))
@@ -236,13 +236,14 @@ class Index:
                       bloom_bits=0, batch=0, n_seeds=0, waves_per_block=0, lanes_per_row=0, mode=0,
                       waves_per_cu=0, learned=0, stop_threshold=0.0, stop_min_hops=0, stop_interval=1,
                       stop_max_hops=0, stop_confirm=0, result_cap=0, stop_late_hops=0, stop_late_interval=0,
+                      entry_late_hops=0,
                       no_alloc=False) -> dict:
         """Search with every buffer resident on the device; returns the timing dict.
         no_alloc: through song_search_device_static (no device allocation; its scratch is a
         static array per GPU, so calls on one GPU must not overlap)."""
         o = N.make_opts(bloom_bits, batch, n_seeds, waves_per_block, lanes_per_row, mode, waves_per_cu,
                         learned, stop_threshold, stop_min_hops, stop_interval, stop_max_hops, stop_confirm,
-                        result_cap, stop_late_hops, stop_late_interval)
+                        result_cap, stop_late_hops, stop_late_interval, entry_late_hops)
         tm = N.SongTiming()
         fn = N.lib().song_search_device_static if no_alloc else N.lib().song_search_device
         N.check(fn(self._h, dq.ptr, nq, k, ef, ctypes.byref(o), d_ids.ptr,
@@ -255,11 +256,12 @@ class Index:
                             d_trace: N.DeviceBuffer | None = None, *, bloom_bits=0, n_seeds=0,
                             waves_per_block=0, lanes_per_row=0, mode=0, waves_per_cu=0, learned=0,
                             stop_threshold=0.0, stop_min_hops=0, stop_interval=1, stop_max_hops=0,
-                            stop_confirm=0, result_cap=0, stop_late_hops=0, stop_late_interval=0) -> dict:
+                            stop_confirm=0, result_cap=0, stop_late_hops=0, stop_late_interval=0,
+                            entry_late_hops=0) -> dict:
         """Per-query k / queue size in one launch (rows of k_max ids)."""
         o = N.make_opts(bloom_bits, 0, n_seeds, waves_per_block, lanes_per_row, mode, waves_per_cu,
                         learned, stop_threshold, stop_min_hops, stop_interval, stop_max_hops, stop_confirm,
-                        result_cap, stop_late_hops, stop_late_interval)
+                        result_cap, stop_late_hops, stop_late_interval, entry_late_hops)
         tm = N.SongTiming()
         N.check(N.lib().song_search_device_mixed(self._h, dq.ptr, nq, d_ks.ptr, d_efs.ptr, k_max, ef_max,
                                                  ctypes.byref(o), d_ids.ptr,

@@ -424,14 +424,15 @@ def main():
         # ends by the stop network or the hop cap, not by the reference's test against the worst
         # of ef results (see include/song.h) -- which lets a short queue search long enough
         # the hop cap bounds the launch's last queries (their hops at full load set its end)
-        # (late consults: every 64 hops from hop 150 on, where only the long searches are left)
-        # (the entry network, learned bits 0 and 3, was measured on this grid too and reached the
-        # recall target only slower than the reference seeds)
-        grid = {"ef": [48, 64], "learned": [2], "stop_threshold": [1.5, 1.75, 2.0, 2.25],
-                "bloom_bits": [1 << 15], "stop_confirm": [1, 2], "stop_max_hops": [400, 450],
-                "result_cap": [16], "stop_min_hops": [16, 32], "stop_interval": [STOP_INTERVAL],
-                "late": [{"stop_late_hops": h, "stop_late_interval": i} for h, i in ((0, 0), (150, 64))],
-                "mode": [1], "waves_per_block": [4]}
+        # (the entry network's seeds in place of the reference ones, learned bits 0 and 3, reached
+        # the recall target only slower; joined late, at hop 50 or 100, they bring back searches
+        # that the reference seeds sent astray -- the hop-capped ones end at recall ~0.24 -- and
+        # the tuner weighs that against their cost)
+        grid = {"ef": [48, 64],
+                "entry": [{"learned": 2}] + [{"learned": 3, "entry_late_hops": h} for h in (50, 100)],
+                "stop_threshold": [1.25, 1.5, 1.75, 2.0, 2.25], "bloom_bits": [1 << 15], "stop_confirm": [1],
+                "stop_max_hops": [400, 450], "result_cap": [16], "stop_min_hops": [16, 32],
+                "stop_interval": [STOP_INTERVAL], "mode": [1], "waves_per_block": [4]}
         given = json.loads(args.learned_config) if args.learned_config else stored.get("learned")
         if given:
             grid = {kk: [v] for kk, v in given.items()}
@@ -453,7 +454,8 @@ def main():
             lhost = host_buffers(args.k, lef, **p)
             best_search = (lef, p)
             lanes, chunks = shape_of(p)
-            lname = f"persistent_kernel<0,{slots_for(lef)},{lanes},{chunks},1>"
+            lmode = 3 if p.get("entry_late_hops", 0) > 0 else 1  # the late-entry instantiation
+            lname = f"persistent_kernel<0,{slots_for(lef)},{lanes},{chunks},{lmode}>"
             lkey = f"{lname} {args.n}x{dim} nq={nq} " + " ".join(f"{kk}={v}" for kk, v in sorted({"ef": lef, **p}.items()))
             learned_roofline = roofline_of(ltr, lkernel_ms, lname, lkey)
             learned_out = {"params": {"ef": lef, **p}, "recall_at_10": round(lrec, 4), "qps": round(lqps, 1),

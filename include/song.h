@@ -147,6 +147,14 @@ typedef struct song_search_opts {
                                 hops instead (0 = never; no reference
                                 counterpart) */
     int32_t stop_late_interval; /* >= 1 when stop_late_hops > 0 */
+    int32_t entry_late_hops; /* with learned bits 0 and 1 (not bit 3): the
+                                search starts at the n_seeds reference seeds
+                                and the entry network's n_seeds seeds join it
+                                at this hop, tested and pushed like a row
+                                before the popped node's row, if the search
+                                is still running (0 = the entry seeds replace
+                                the reference ones from the start; no
+                                reference counterpart) */
 } song_search_opts;
 
 enum { SONG_MODE_PLAIN = 0, SONG_MODE_PERSISTENT = 1 };

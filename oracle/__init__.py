@@ -82,7 +82,8 @@ def lib():
                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, _I64, ctypes.c_int,
                                         ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P,
                                         ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int,
-                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P, _P]
+                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        _P, _P, _P]
         L.or_search_learned.restype = ctypes.c_int
         L.or_stop_logits.argtypes = [_P, ctypes.c_float, _P, ctypes.c_int, _P]
         L.or_learned_trace.argtypes = [_P, _P, _I64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P,
@@ -169,9 +170,11 @@ def search(data, adj, queries, k, ef, *, block, metric="l2", bloom_bits=1 << 14,
 
 def search_learned(data, adj, queries, k, ef, *, block, stop_w, dist_scale, threshold, min_hops, interval,
                    confirm=1, max_hops=0, result_cap=0, late_hops=0, late_interval=0, metric="l2",
-                   bloom_bits=1 << 14, n_seeds=4, group=8, entries=None):
+                   bloom_bits=1 << 14, n_seeds=4, group=8, entries=None, entry_late_hops=0):
     """The search with the learned stop network (the MI355X kernel's learned bit 1) re-executed on the
-    CPU: stop_w is the flat network (w1 [64 x 8], b1, w2 [64 x 64], b2, w3, then b3).
+    CPU: stop_w is the flat network (w1 [64 x 8], b1, w2 [64 x 64], b2, w3, then b3).  entries
+    [nq, n] replace the reference seeds, or with entry_late_hops > 0 join the search at that hop
+    (which starts at the n reference seeds).
     Returns (ids [nq, k] best-first, dists [nq, k], trace [nq, 12])."""
     m = METRIC[metric]
     if m == 2:
@@ -195,7 +198,8 @@ def search_learned(data, adj, queries, k, ef, *, block, stop_w, dist_scale, thre
     rc = lib().or_search_learned(_ptr(data), _ptr(adj), n, dim, block, m, _ptr(queries), nq, k, ef, bloom_bits,
                                  n_seeds, group, _ptr(entries), ne, int(max_hops), int(result_cap), _ptr(w),
                                  float(w[4800]), float(dist_scale), float(threshold), int(min_hops), int(interval),
-                                 int(confirm), int(late_hops), int(late_interval), _ptr(ids), _ptr(d), _ptr(tr))
+                                 int(confirm), int(late_hops), int(late_interval), int(entry_late_hops), _ptr(ids),
+                                 _ptr(d), _ptr(tr))
     if rc != 0:
         raise ValueError("or_search_learned rejected its arguments")
     return ids, d, tr

@@ -390,6 +390,8 @@ typedef struct or_stop {
      * the 8 features and the k-th best distance into rec [max_evals x 9] */
     float *rec;
     int max_evals, n_rec;
+    int entry_late_hops; /* > 0: the search starts at the reference seeds and the
+                            entry list joins at this hop (the kernel's late entry) */
 } or_stop;
 
 /* The k smallest distances seen so far, ascending (k <= 64): a new distance
@@ -485,8 +487,8 @@ static void peak_update(int32_t *tr, const or_work *w)
 
 /* One query; entry = list of seed ids (NULL: the reference's seeds). */
 static void search_one(const or_space *sp, const or_params *p, or_work *w, const void *q,
-                       const int32_t *entry, int n_entry, int32_t *ids_out, float *d_out,
-                       int32_t *tr)
+                       const int32_t *entry, int n_entry, const int32_t *late, int n_late, int late_hop,
+                       int32_t *ids_out, float *d_out, int32_t *tr)
 {
     memset(tr, 0, TR_N * sizeof(int32_t));
     memset(&w->topk, 0, sizeof w->topk);
@@ -576,6 +578,24 @@ static void search_one(const or_space *sp, const or_params *p, or_work *w, const
                 }
             }
         }
+        if (late && tr[TR_EXPANDED] == late_hop) {
+            /* the entry list joins: tested and pushed in order, before the popped node's row */
+            for (int s = 0; s < n_late; s++) {
+                int ep = late[s];
+                if (ep < 0 || ep >= p->n) continue;
+                if (visited_test_set(p, w, ep)) continue;
+                float dp = space_dist(sp, q, ep);
+                int fa = set_push(&w->cand, dp, ep);
+                int ra = set_push(&w->res, dp, ep);
+                if (st && topk_insert(&w->topk, dp)) w->last_insert_hop = tr[TR_EXPANDED];
+                tr[TR_VISITED]++;
+                tr[TR_CANDIDATE]++;
+                tr[TR_DEVALS]++;
+                tr[TR_FPUSH] += fa;
+                tr[TR_RUPD] += ra;
+                peak_update(tr, w);
+            }
+        }
         for (int j = 0; j < p->block; j++) {
             tr[TR_SLOTS]++;
             int nb = p->adj[(int64_t)cn * p->block + j];
@@ -656,12 +676,14 @@ int or_search_learned(const void *data, const int32_t *adj, int64_t n, int dim, 
                       const void *queries, int nq, int k, int ef, int64_t bloom_bits, int n_seeds, int G,
                       const int32_t *entries, int n_entry, int max_hops, int result_cap, const float *stop_w,
                       float stop_b3, float dist_scale, float threshold, int min_hops, int interval, int confirm,
-                      int late_hops, int late_interval, int32_t *ids_out, float *dists_out, int32_t *trace_out)
+                      int late_hops, int late_interval, int entry_late_hops, int32_t *ids_out, float *dists_out,
+                      int32_t *trace_out)
 {
     if (!stop_w || k > 64 || interval < 1 || min_hops < 0 || confirm < 1 || !(dist_scale > 0.f)) return -1;
-    if (late_hops < 0 || (late_hops > 0 && late_interval < 1)) return -1;
+    if (late_hops < 0 || (late_hops > 0 && late_interval < 1) || entry_late_hops < 0) return -1;
+    if (entry_late_hops > 0 && !entries) return -1;
     or_stop st = {stop_w, stop_b3, dist_scale, threshold, min_hops, interval, confirm, late_hops, late_interval,
-                  NULL, 0, 0};
+                  NULL, 0, 0, entry_late_hops};
     return search_all(data, adj, n, dim, block, metric, queries, nq, k, ef, bloom_bits, n_seeds, G, 0, 0, 0,
                       entries, n_entry, max_hops, result_cap, &st, ids_out, dists_out, trace_out);
 }
@@ -736,7 +758,11 @@ static int search_all(const void *data, const int32_t *adj, int64_t n, int dim, 
             for (int s = 0; s < ne; s++) seeds[s] = (int32_t)or_seed_id(s, n, ne);
         }
         const char *q = (const char *)queries + (int64_t)qi * qstride * esz;
-        search_one(&sp, &p, &w, q, seeds, ne, ids_out + (int64_t)qi * k,
+        const int late_hop = stop && entries ? stop->entry_late_hops : 0;
+        if (late_hop > 0) /* reference seeds first, the query's entry list at late_hop */
+            for (int s = 0; s < ne; s++) seeds[s] = (int32_t)or_seed_id(s, n, ne);
+        search_one(&sp, &p, &w, q, seeds, ne, late_hop > 0 ? entries + (int64_t)qi * ne : NULL, ne, late_hop,
+                   ids_out + (int64_t)qi * k,
                    dists_out ? dists_out + (int64_t)qi * k : NULL, trace_out + (int64_t)qi * TR_N);
     }
     free(seeds);

@@ -413,6 +413,37 @@ def test_stop_network_that_fires_matches_oracle(song, oracle, setup, variant, mo
     assert fired_some, "no threshold stopped a part of the queries"
 
 
+@pytest.mark.parametrize("hop", [6, 20])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_late_entry_matches_oracle(song, oracle, setup, hop, mode):
+    """entry_late_hops: the search starts at the reference seeds and the entry network's 4 best
+    pivots join at that hop; equal to the oracle run with those pivots (the planning pass's
+    projection sums in another order than numpy's, so a near-tie of two pivot scores can pick
+    another list for a few queries)."""
+    base, q, adj, idx, model = setup
+    kw = dict(stop_threshold=2.0, stop_min_hops=8, stop_interval=4, stop_max_hops=200, result_cap=16)
+    r = idx.search(q, 10, 48, learned=3, entry_late_hops=hop, mode=mode, **kw)
+    scores = learned.mlp_forward(model.entry_w, model.project(q, 0).astype(np.float32), 16, 64)
+    seeds = model.pivots[np.argsort(-scores, axis=1, kind="stable")[:, :4]]
+    oi, od, ot = oracle.search_learned(
+        base, adj, q, 10, 48, block=32, group=8, bloom_bits=1 << 14, stop_w=model.stop_w,
+        dist_scale=model.dist_scale, threshold=2.0, min_hops=8, interval=4, max_hops=200, result_cap=16,
+        entries=seeds, entry_late_hops=hop)
+    same = (r.ids == oi).all(1) & (r.trace == ot).all(1) & (r.dists.view(np.uint32) == od.view(np.uint32)).all(1)
+    assert same.mean() > 0.97, same.mean()
+    # the join changed some searches: without it they run from the reference seeds alone
+    plain_seeds = idx.search(q, 10, 48, learned=2, mode=mode, **kw)
+    assert (plain_seeds.ids != r.ids).any()
+    assert (r.trace[:, 1] >= 0).all()
+
+
+def test_late_entry_needs_the_entry_network_and_the_stop(song, setup):
+    base, q, adj, idx, model = setup
+    for bits in (2, 1, 11):
+        with pytest.raises(song.SongError, match="entry_late_hops"):
+            idx.search(q[:4], 10, 48, learned=bits, entry_late_hops=10, stop_max_hops=100)
+
+
 @pytest.mark.parametrize("late", [(0, 0), (23, 7)])
 def test_trace_features_match_oracle(song, oracle, setup, late):
     """The features the stop network sees (trace mode records them) equal the oracle's, bit for bit,

@@ -401,6 +401,42 @@ def test_learned_trace_features_follow_their_definitions(oracle, small_graph):
         assert r[-1, 8] >= fin[i]
 
 
+def test_late_entry_after_the_end_changes_nothing(oracle, small_graph):
+    """An entry list that would join after every search has ended leaves the search that starts
+    at the reference seeds; joining lists made of those seeds (already visited) changes nothing
+    either, at any hop."""
+    base, q, adj = small_graph
+    w = _stop_weights(7)
+    kw = dict(block=16, group=8, stop_w=w, dist_scale=50.0, threshold=1.0, min_hops=6, interval=3, max_hops=80,
+              result_cap=16)
+    ref = oracle.search_learned(base, adj, q, 10, 48, **kw)
+    rng = np.random.default_rng(4)
+    other = rng.integers(0, base.shape[0], size=(q.shape[0], 4)).astype(np.int32)
+    late = oracle.search_learned(base, adj, q, 10, 48, entries=other, entry_late_hops=100000, **kw)
+    seeds = np.tile(np.array([(p * base.shape[0]) // 4 for p in range(4)], np.int32), (q.shape[0], 1))
+    same = oracle.search_learned(base, adj, q, 10, 48, entries=seeds, entry_late_hops=5, **kw)
+    for a, b, c in zip(ref, late, same):
+        np.testing.assert_array_equal(a, b)
+        np.testing.assert_array_equal(a, c)
+
+
+def test_late_entry_of_the_true_neighbours(oracle, song):
+    """Joining each query's 4 true nearest neighbours (float data: no ties) at hop 3 puts exactly
+    them, in order, at the head of its results, and adds them to the visited count of hop 3."""
+    rng = np.random.default_rng(8)
+    base = rng.standard_normal((3000, 16)).astype(np.float32)
+    q = rng.standard_normal((30, 16)).astype(np.float32)
+    adj = song.nsw_build(base, block=16, exact_order=False)
+    d = ((q[:, None, :].astype(np.float64) - base[None, :, :]) ** 2).sum(-1)
+    nn = np.argsort(d, axis=1)[:, :4].astype(np.int32)
+    w = _stop_weights(9)
+    kw = dict(block=16, group=8, stop_w=w, dist_scale=20.0, threshold=np.inf, min_hops=6, interval=3, max_hops=40,
+              result_cap=16)
+    ids, _, tr = oracle.search_learned(base, adj, q, 10, 48, entries=nn, entry_late_hops=3, **kw)
+    np.testing.assert_array_equal(ids[:, :4], nn)
+    assert (tr[:, 11] == 3).all()  # the network never fires; every search runs to the hop cap
+
+
 def test_learned_trace_late_schedule(oracle, small_graph):
     """From late_hops on the consults come every late_interval hops: hops 5, 8, ..., 20, then
     every 7 (the consult at 20 is the first at or after late_hops)."""
