@@ -948,6 +948,7 @@ int song_learn_trace(song_index* ix, const void* d_queries, int32_t nq, int32_t 
     if (opts) o = *opts;
     else std::memset(&o, 0, sizeof o);
     o.learned = (o.learned & 1) | 2;  // trace mode rides on the stop path
+    if (o.entry_late_hops != 0) return fail(SONG_ERR_ARG, "song_learn_trace records searches without entry_late_hops");
     Plan p;
     int rc = make_plan(ix, nq, k, ef, &o, &p);
     if (rc) return rc;

@@ -366,7 +366,7 @@ int song_index_set_model(song_index* index, const song_learned_model* model);
  * every consulted hop (opts->stop_min_hops / stop_interval), the 8 features
  * and the current k-th best distance: d_trace [nq x max_evals x 9] (device),
  * d_count [nq] rows written, d_final_dk [nq] the k-th best at the end.
- * Entry per opts->learned bit 0.  k <= 64. */
+ * Entry per opts->learned bit 0 (opts->entry_late_hops must be 0).  k <= 64. */
 int song_learn_trace(song_index* index, const void* d_queries, int32_t nq, int32_t k, int32_t ef,
                      const song_search_opts* opts, int32_t max_evals, float* d_trace, int32_t* d_count,
                      float* d_final_dk);

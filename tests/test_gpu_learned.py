@@ -442,6 +442,16 @@ def test_late_entry_needs_the_entry_network_and_the_stop(song, setup):
     for bits in (2, 1, 11):
         with pytest.raises(song.SongError, match="entry_late_hops"):
             idx.search(q[:4], 10, 48, learned=bits, entry_late_hops=10, stop_max_hops=100)
+    # trace mode records searches without it
+    N = song._native
+    L = learned._bind()
+    import ctypes
+    dq = idx.queries_to_device(q[:4])
+    bufs = N.DeviceBuffer(4 * 8 * 9 * 4), N.DeviceBuffer(16), N.DeviceBuffer(16)
+    o = N.make_opts(learned=1, entry_late_hops=10)
+    with pytest.raises(song.SongError, match="entry_late_hops"):
+        N.check(L.song_learn_trace(idx.handle, dq.ptr, 4, 10, 48, ctypes.byref(o), 8, *(b.ptr for b in bufs)),
+                "song_learn_trace")
 
 
 @pytest.mark.parametrize("late", [(0, 0), (23, 7)])
