@@ -395,9 +395,12 @@ def main():
                                  f"{pname} {args.n}x{dim} nq={nq} queue={ef} k={args.k} "
                                  f"bloom_bits={plain_params.get('bloom_bits', 1 << 14)}")
 
-    # ---------------- persistent kernel alone (no learning), for the record
-    pers_qps, _, pers_kernel_ms, _ = timed(args.k, ef, mode=1)
-    log(f"persistent (no learning): queue {ef}, {pers_qps:.0f} queries/s")
+    # ---------------- persistent kernel alone (no learning), for the record: the plain search's
+    # queue, filter and row shape (the same results), launched as a device work queue
+    pers_params = {kk: v for kk, v in plain_params.items() if kk != "waves_per_block"}
+    pers_params["mode"] = 1
+    pers_qps, _, pers_kernel_ms, _ = timed(args.k, ef, **pers_params)
+    log(f"persistent (no learning): queue {ef} {pers_params}, {pers_qps:.0f} queries/s")
 
     # ---------------- learned + persistent
     learned_out = None
@@ -513,7 +516,7 @@ def main():
                              "qps": round(plain_qps, 1), "ms_per_step": round(plain_ms_step, 4),
                              "kernel_ms": round(plain_kernel_ms, 4), "kernel_ms_std": round(plain_kernel_std, 4),
                              "qps_kernel": round(nq / (plain_kernel_ms / 1e3), 1), "host_buffers": plain_host},
-                   "persistent_no_learning": {"queue": ef, "qps": round(pers_qps, 1),
+                   "persistent_no_learning": {"queue": ef, "params": pers_params, "qps": round(pers_qps, 1),
                                               "kernel_ms": round(pers_kernel_ms, 4)},
                    "learned_vs_plain": (round(learned_out["qps"] / plain_qps, 3)
                                         if learned_out and "qps" in learned_out else None),
