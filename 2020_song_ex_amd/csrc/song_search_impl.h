@@ -624,6 +624,11 @@ __device__ __forceinline__ void distance_passes(const float4* __restrict__ data,
     }
 }
 
+// Passes in flight: two for rows shared by several lane groups; with one row
+// per pass (64 lanes per row: long rows, where few waves run) up to 8, held to
+// about 128 VGPRs of row data.
+__host__ __device__ constexpr int rows_in_flight(int g, int qch) { return g == 64 && qch <= 4 ? 8 : 2; }
+
 // NP passes at a time while that many remain, then one at a time (a row's sum
 // does not depend on which passes share a step).  `shadow` runs once, behind
 // the first step's loads.
@@ -635,6 +640,29 @@ __device__ __forceinline__ void float_distances_v(const float4* __restrict__ dat
     const int passes = (F + NG - 1) / NG;
     auto none = []() {};
     int p0 = 0;
+    if constexpr (NP == 8) {
+        // one row per pass (long rows): the largest step of 8, 4, 2 or 1 passes that fits,
+        // then steps of 8, 4, 2, 1 for the rest, so a hop's rows take at most four round trips
+        const int first = passes >= 8 ? 8 : passes >= 4 ? 4 : passes >= 2 ? 2 : 1;
+        if (first == 8) distance_passes<METRIC, G, QCH, 8, FULL, OFF32>(data, d4, q, F, 0, stage_id, stage_d, lane, shadow);
+        else if (first == 4) distance_passes<METRIC, G, QCH, 4, FULL, OFF32>(data, d4, q, F, 0, stage_id, stage_d, lane, shadow);
+        else if (first == 2) distance_passes<METRIC, G, QCH, 2, FULL, OFF32>(data, d4, q, F, 0, stage_id, stage_d, lane, shadow);
+        else distance_passes<METRIC, G, QCH, 1, FULL, OFF32>(data, d4, q, F, 0, stage_id, stage_d, lane, shadow);
+        p0 = first;
+        for (; passes - p0 >= 8; p0 += 8)
+            distance_passes<METRIC, G, QCH, 8, FULL, OFF32>(data, d4, q, F, p0, stage_id, stage_d, lane, none);
+        if (passes - p0 >= 4) {
+            distance_passes<METRIC, G, QCH, 4, FULL, OFF32>(data, d4, q, F, p0, stage_id, stage_d, lane, none);
+            p0 += 4;
+        }
+        if (passes - p0 >= 2) {
+            distance_passes<METRIC, G, QCH, 2, FULL, OFF32>(data, d4, q, F, p0, stage_id, stage_d, lane, none);
+            p0 += 2;
+        }
+        if (passes - p0 >= 1)
+            distance_passes<METRIC, G, QCH, 1, FULL, OFF32>(data, d4, q, F, p0, stage_id, stage_d, lane, none);
+        return;
+    }
     if (passes >= NP) {
         distance_passes<METRIC, G, QCH, NP, FULL, OFF32>(data, d4, q, F, 0, stage_id, stage_d, lane, shadow);
         for (p0 = NP; p0 + NP <= passes; p0 += NP)
@@ -886,7 +914,8 @@ __device__ __forceinline__ void search_query(const SearchArgs& P, int qid, int l
         if constexpr (METRIC == METRIC_HAMMING) {
             hamming_distances((const ulonglong2*)P.data, qc, F, stage_id, stage_d, lane, shadow);
         } else {
-            float_distances<METRIC, G, QCH, 2>((const float4*)P.data, d4, P.off32 != 0, qf, F, stage_id, stage_d, lane,
+            float_distances<METRIC, G, QCH, rows_in_flight(G, QCH)>((const float4*)P.data, d4, P.off32 != 0, qf, F,
+                                                                   stage_id, stage_d, lane,
                                                shadow);
         }
         wave_lds_sync();
