@@ -888,13 +888,14 @@ def workload_c5(args, threads, share):
     idx = song.Index(base, adj, metric="ip")
     nq = queries.shape[0]
     gt, _, _ = idx.bruteforce(queries, 100)
-    learned.fit(idx, base, train, k=10, ef=256, stop_min_hops=16, stop_interval=8, max_evals=192,
-                entry_epochs=10, stop_epochs=4, cost_threshold=3.0, log=log)
+    # the stop network trained on searches from the reference seeds, consulted every 32 hops
+    learned.fit(idx, base, train, k=10, ef=256, stop_min_hops=16, stop_interval=32, max_evals=192,
+                entry_epochs=10, stop_epochs=4, cost_threshold=3.0, log=log, trace_entry=False)
     rng = np.random.default_rng(5)
     ks_all = rng.choice([1, 10, 100], size=nq).astype(np.int32)
     efs_all = rng.choice([64, 256], size=nq).astype(np.int32)
     # the learned search serves k <= 64, so its result set holds 64 (k = 100 falls back to plain)
-    search = dict(learned=2, stop_threshold=2.0, stop_min_hops=16, stop_interval=8, stop_max_hops=400,
+    search = dict(learned=2, stop_threshold=2.0, stop_min_hops=16, stop_interval=32, stop_max_hops=400,
                   result_cap=64, bloom_bits=1 << 15, waves_per_block=4, mode=0)
     curve = []
     for rate in (50_000.0, 200_000.0, 500_000.0):
