@@ -216,6 +216,8 @@ int make_plan(const song_index* ix, int32_t nq, int32_t k, int32_t ef, const son
     if (o->entry_late_hops < 0) return fail(SONG_ERR_ARG, "entry_late_hops must be >= 0");
     if (o->entry_late_hops > 0 && ((o->learned & 3) != 3 || (o->learned & 8)))
         return fail(SONG_ERR_ARG, "entry_late_hops needs learned bits 0 and 1 without bit 3");
+    if (o->entry_late_hops > 0 && p->S != 1)
+        return fail(SONG_ERR_ARG, "entry_late_hops needs a queue of at most 64 after ef = max(ef, k)");
     if (o->learned) {
         if (!ix->has_model) return fail(SONG_ERR_STATE, "learned search needs a model (song_index_set_model)");
         if (k > 64) return fail(SONG_ERR_ARG, "learned search supports k <= 64");

@@ -1361,7 +1361,9 @@ static hipError_t launch_one(const SearchArgs& a, const LaunchShape& ls, hipStre
     if (!a.learned) return launch_impl<METRIC, S, G, QCH, 0>(a, ls, st);
     if constexpr (QCH <= 4) {
         if (a.L.trace) return launch_impl<METRIC, S, G, QCH, 2>(a, ls, st);
-        if (a.L.entry_late_hops > 0 && a.entries) return launch_impl<METRIC, S, G, QCH, 3>(a, ls, st);
+        // the late entry is built for the short queues of the learned search only
+        if constexpr (S == 1)
+            if (a.L.entry_late_hops > 0 && a.entries) return launch_impl<METRIC, S, G, QCH, 3>(a, ls, st);
         return launch_impl<METRIC, S, G, QCH, 1>(a, ls, st);
     }
     return hipErrorInvalidValue;

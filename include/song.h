@@ -147,7 +147,8 @@ typedef struct song_search_opts {
                                 hops instead (0 = never; no reference
                                 counterpart) */
     int32_t stop_late_interval; /* >= 1 when stop_late_hops > 0 */
-    int32_t entry_late_hops; /* with learned bits 0 and 1 (not bit 3): the
+    int32_t entry_late_hops; /* with learned bits 0 and 1 (not bit 3) and a
+                                queue of at most 64 after ef = max(ef, k): the
                                 search starts at the n_seeds reference seeds
                                 and the entry network's n_seeds seeds join it
                                 at this hop, tested and pushed like a row

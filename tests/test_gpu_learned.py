@@ -442,6 +442,8 @@ def test_late_entry_needs_the_entry_network_and_the_stop(song, setup):
     for bits in (2, 1, 11):
         with pytest.raises(song.SongError, match="entry_late_hops"):
             idx.search(q[:4], 10, 48, learned=bits, entry_late_hops=10, stop_max_hops=100)
+    with pytest.raises(song.SongError, match="entry_late_hops"):
+        idx.search(q[:4], 10, 128, learned=3, entry_late_hops=10, stop_max_hops=100)  # queue over 64
     # trace mode records searches without it
     N = song._native
     L = learned._bind()
