@@ -347,20 +347,8 @@ def main():
 
     slots_for = _slots
 
-    # lanes per row and 16-byte chunks per lane, as the library picks them for dim (or for the
-    # given lanes_per_row): the kernels' template arguments G and QCH
-    d4 = (dim + 3) // 4
-
     def shape_of(params):
-        lanes = params.get("lanes_per_row") or 0
-        if not lanes:
-            lanes = 2
-            while lanes < 64 and lanes * 2 < d4:
-                lanes *= 2
-        chunks = 2
-        while lanes * chunks < d4:
-            chunks *= 2
-        return lanes, chunks
+        return _row_shape(dim, params.get("lanes_per_row") or 0)
 
     def roofline_of(trace_rows, kernel_ms, kname, key):
         """Algorithmic bytes of one launch / its HIP-event duration.  Beside it, from the committed
@@ -667,6 +655,21 @@ def config1_leg(song, datasets, args, threads):
 
 
 # ---------------------------------------------------------------- the other workloads
+
+def _row_shape(dim, lanes_per_row=0):
+    """Lanes per row and 16-byte chunks per lane, as the library picks them for dim (or for the
+    given lanes per row): the kernels' template arguments G and QCH."""
+    d4 = (dim + 3) // 4
+    lanes = lanes_per_row
+    if not lanes:
+        lanes = 2
+        while lanes < 64 and lanes * 2 < d4:
+            lanes *= 2
+    chunks = 2
+    while lanes * chunks < d4:
+        chunks *= 2
+    return lanes, chunks
+
 
 def _slots(q):
     """Set slots per lane of a queue of q entries, as the library picks them (the kernels'
